@@ -1,0 +1,110 @@
+"""The FP64 oracle against the golden fixtures written from the reference's own
+3-D core (tests/golden/make_golden.py).  Integer maps and FP64 tables must be
+bit-identical; the 40-step field run too (same association order, no FMA)."""
+import hashlib
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import cases
+from cpsfdtd_amd.case import REF_ZEROHEAP
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return dict(np.load(os.path.join(GOLD, name), allow_pickle=False))
+
+
+def _shipped_case(tmp_path, structure, pml, flags=REF_ZEROHEAP):
+    from cpsfdtd_amd.case import CaseWriter
+    d = str(tmp_path)
+    for f in ("material.txt", structure) + (("CFS_PML.txt",) if pml else ()):
+        shutil.copy(os.path.join(GOLD, "inputs", f), os.path.join(d, "structure.txt" if f == structure else f))
+    w = CaseWriter(d)
+    w.case.pml = pml
+    return w.run(1, flags=flags)
+
+
+@pytest.mark.parametrize("tag,structure,pml", [("structure", "structure.txt", True), ("shi1", "Shi1.txt", False)])
+def test_shipped_structure_id_map(oracle_lib, tmp_path, tag, structure, pml):
+    g = _load("ref_shipped.npz")
+    o = oracle_lib.OracleSim(_shipped_case(tmp_path, structure, pml))
+    ids = o.id_map()
+    assert list(o.N) == list(g[tag + "_dims"][:3])
+    assert np.array_equal(np.array(o.d + (o.dt,)), g[tag + "_d"])
+    assert hashlib.sha256(ids.astype("<i2").tobytes()).hexdigest() == bytes(g[tag + "_sha256"]).decode()
+    assert np.array_equal(np.bincount(ids.ravel().astype(np.int64), minlength=16), g[tag + "_hist"])
+    N = o.N
+    assert np.array_equal(ids[N[0] // 2], g[tag + "_plane_x"])
+    assert np.array_equal(ids[:, :, N[2] // 2], g[tag + "_plane_z"])
+    assert np.array_equal(np.concatenate(o.coefs()), g[tag + "_coef"])
+    assert np.array_equal(np.concatenate([o.stretch(w) for w in range(12)]), g[tag + "_stretch"])
+    if pml:
+        lens, pv = g[tag + "_cpml_len"], g[tag + "_cpml"]
+        off = 0
+        for w in range(12):
+            l = int(lens[w])
+            b, c = o.cpml(w)
+            assert np.array_equal(b, pv[off:off + l]) and np.array_equal(c, pv[off + l:off + 2 * l]), w
+            off += 2 * l
+
+
+def test_shipped_cpml_last_h_coefficient_is_defined(oracle_lib, tmp_path):
+    """F1: the reference leaves the innermost H coefficient unset; the oracle
+    fills it from the same grading formula, so it differs from the zero-heap
+    reference value only at that index and follows the grading trend."""
+    g = _load("ref_shipped.npz")
+    o = oracle_lib.OracleSim(_shipped_case(tmp_path, "structure.txt", True, flags=0))
+    lens, pv = g["structure_cpml_len"], g["structure_cpml"]
+    off = 0
+    for w in range(12):
+        l = int(lens[w])
+        if w & 1:
+            b, c = o.cpml(w)
+            side = (w // 2) & 1
+            last = l - 1
+            assert pv[off + last] == 0.0 and pv[off + l + last] == 0.0  # zero-filled heap in the fixture
+            assert np.array_equal(b[:last], pv[off:off + last])
+            assert 0.0 < b[last] < 1.0 and c[last] < 0.0
+            # b grows toward the interior on the first side (index up), and
+            # toward the interior on the second side too (index down)
+            assert (b[last] > b[last - 1]) if side == 0 else (b[last] < b[last - 1])
+        off += 2 * l
+
+
+@pytest.mark.parametrize("tag,kw", [("k0zero", {}), ("k0set", {"zs": 120e-9})])
+def test_mixed_tables(oracle_lib, tmp_path, tag, kw):
+    g = _load("ref_mixed.npz")
+    case = cases.mixed(str(tmp_path), **kw)
+    o = oracle_lib.OracleSim(case)
+    p = tag + "_"
+    assert np.array_equal(o.id_map().ravel(), g[p + "id"])
+    assert np.array_equal(np.concatenate(o.coefs()), g[p + "coef"])
+    assert np.array_equal(np.concatenate([o.stretch(w) for w in range(12)]), g[p + "stretch"])
+    ijk, tab = o.tfsf_points()
+    rij = g[p + "tfsf_ijk"].reshape(-1, 3)
+    rt = g[p + "tfsf_tab"].reshape(-1, 8)
+    assert np.array_equal(ijk[:, :3], rij)
+    m1, m2 = (ijk[:, 3] & 1) > 0, (ijk[:, 3] & 2) > 0
+    assert np.array_equal(tab[m1], rt[m1][:, 0:4])
+    assert np.array_equal(tab[m2], rt[m2][:, 4:8])
+    e, h = o.tfsf_wave()
+    n = len(g[p + "tfsf_einc"])
+    assert np.array_equal(e[:n], g[p + "tfsf_einc"]) and np.array_equal(h[:n], g[p + "tfsf_hinc"])
+    assert np.array_equal(o.tfsf_vec(), g[p + "tfsf_vec"][:12])
+    for q in range(2):
+        assert np.array_equal(o.port_points(q).ravel(), g[p + "port%d" % q])
+
+
+def test_tiny_run_bit_exact(oracle_lib, tmp_path):
+    g = _load("ref_tiny.npz")
+    case = cases.tiny(str(tmp_path))
+    o = oracle_lib.OracleSim(case)
+    o.step(int(g["nt"][0]))
+    for c, nm in enumerate(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")):
+        assert np.array_equal(o.field(c).ravel(), g[nm]), nm
+    assert np.array_equal(o.port_samples(0), g["port0"])
+    assert np.abs(g["Ez"]).max() > 1e-3  # the run actually carries a field
