@@ -1,0 +1,349 @@
+"""MI355X-native 3-D FDTD engine (cpsFDTD's time-stepping loop on HIP/gfx950).
+
+Python-side host interface over the C ABI in include/cpsfdtd.h
+(libcpsfdtd.so).  The engine itself is C++ and HIP; this module only loads
+the library, fills the configuration struct and moves arrays.  There is no
+CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .case import Case  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcpsfdtd.so")
+
+CFD_F32 = 32
+CFD_F64 = 64
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class _Config(C.Structure):
+    _fields_ = [
+        ("material", C.c_char_p), ("structure", C.c_char_p), ("pml", C.c_char_p), ("tfsf", C.c_char_p),
+        ("dl", C.c_char_p), ("gain", C.c_char_p), ("source", C.c_char_p),
+        ("n_ports", C.c_int), ("port_files", C.POINTER(C.c_char_p)), ("port_kinds", C.POINTER(C.c_int)),
+        ("n_src", C.c_int), ("src_ijkc", C.POINTER(C.c_int)), ("src_amp", C.POINTER(C.c_double)),
+        ("src_phase", C.POINTER(C.c_double)),
+        ("nt", C.c_int), ("n_freq", C.c_int), ("freqs", C.POINTER(C.c_double)), ("dft_stride", C.c_int),
+        ("record_ports", C.c_int), ("periodic", C.c_int), ("flags", C.c_uint), ("precision", C.c_int),
+    ]
+
+
+# every entry point declared in include/cpsfdtd.h: name -> (argtypes, restype)
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+SIGNATURES = {
+    "cfd_last_error": ([], C.c_char_p),
+    "cfd_device_info": ([C.c_char_p, C.c_int, _ip, C.POINTER(C.c_size_t)], C.c_int),
+    "cfd_create": ([C.POINTER(_Config), C.POINTER(_vp)], C.c_int),
+    "cfd_destroy": ([_vp], C.c_int),
+    "cfd_info": ([_vp, _ip, _dp, _ip], C.c_int),
+    "cfd_device_bytes": ([_vp], C.c_size_t),
+    "cfd_step": ([_vp, C.c_int], C.c_int),
+    "cfd_steps_done": ([_vp], C.c_int),
+    "cfd_sync": ([_vp], C.c_int),
+    "cfd_basic_h": ([_vp], C.c_int),
+    "cfd_basic_e": ([_vp], C.c_int),
+    "cfd_cpml_h": ([_vp], C.c_int),
+    "cfd_cpml_e": ([_vp], C.c_int),
+    "cfd_tfsf_h": ([_vp, C.c_int], C.c_int),
+    "cfd_tfsf_e": ([_vp, C.c_int], C.c_int),
+    "cfd_dl_e": ([_vp], C.c_int),
+    "cfd_gain_pol": ([_vp], C.c_int),
+    "cfd_gain_e": ([_vp], C.c_int),
+    "cfd_gain_pop": ([_vp], C.c_int),
+    "cfd_sources": ([_vp, C.c_int], C.c_int),
+    "cfd_ports_sample": ([_vp, C.c_int], C.c_int),
+    "cfd_advance": ([_vp], C.c_int),
+    "cfd_read_field": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_write_field": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_read_id": ([_vp, C.POINTER(C.c_int16)], C.c_int),
+    "cfd_port_npts": ([_vp, C.c_int], C.c_int),
+    "cfd_port_points": ([_vp, C.c_int, _ip], C.c_int),
+    "cfd_port_samples": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_port_dft": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_port_intensity": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_gain_read": ([_vp, _dp], C.c_int),
+    "cfd_model_build": ([C.POINTER(_Config), C.POINTER(_vp)], C.c_int),
+    "cfd_model_free": ([_vp], C.c_int),
+    "cfd_model_last_error": ([], C.c_char_p),
+    "cfd_model_dims": ([_vp, _ip, _dp], C.c_int),
+    "cfd_model_id": ([_vp, C.POINTER(C.c_int16)], C.c_int),
+    "cfd_model_table": ([_vp, C.c_char_p, _dp], C.c_int),
+    "cfd_model_tfsf": ([_vp, C.c_int, _ip, _dp, _ip], C.c_int),
+    "cfd_model_port": ([_vp, C.c_int, _ip], C.c_int),
+    "cfd_model_cells": ([_vp, C.c_int, _ip], C.c_int),
+    "cfd_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
+    "cfd_set_timing": ([_vp, C.c_int], C.c_int),
+}
+
+_lib = None
+
+
+def library(path: str = LIB_PATH):
+    """Load libcpsfdtd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise EngineError("libcpsfdtd.so not built: run `python -m cpsfdtd_amd.build` (%s)" % path)
+        L = C.CDLL(path)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise EngineError(library().cfd_last_error().decode())
+    return rc
+
+
+def _enc(s):
+    return s.encode() if s else None
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    ncu = C.c_int(0)
+    mem = C.c_size_t(0)
+    _check(library().cfd_device_info(name, 256, C.byref(ncu), C.byref(mem)))
+    return {"name": name.value.decode(), "cu": ncu.value, "hbm_bytes": mem.value}
+
+
+def make_config(case, precision=CFD_F64):
+    """Fill the C configuration struct from a Case; returns (config, keepalive)."""
+    f = case.files
+    keep = []
+    cfg = _Config()
+    for k in ("material", "structure", "pml", "tfsf", "dl", "gain", "source"):
+        setattr(cfg, k, _enc(f[k]))
+    np_ = len(case.ports)
+    pf = (C.c_char_p * max(np_, 1))(*[_enc(p) for p, _ in case.ports])
+    pk = np.array([k for _, k in case.ports] or [0], dtype=np.int32)
+    src = np.array([s[:4] for s in case.sources] or [[0, 0, 0, 0]], dtype=np.int32).reshape(-1)
+    amp = np.array([s[4] for s in case.sources] or [0.0], dtype=np.float64)
+    ph = np.array([s[5] for s in case.sources] or [0.0], dtype=np.float64)
+    fr = np.array(case.freqs or [0.0], dtype=np.float64)
+    keep += [pf, pk, src, amp, ph, fr]
+    cfg.n_ports = np_
+    cfg.port_files = pf
+    cfg.port_kinds = pk.ctypes.data_as(_ip)
+    cfg.n_src = len(case.sources)
+    cfg.src_ijkc = src.ctypes.data_as(_ip)
+    cfg.src_amp = amp.ctypes.data_as(_dp)
+    cfg.src_phase = ph.ctypes.data_as(_dp)
+    cfg.nt = case.nt
+    cfg.n_freq = len(case.freqs)
+    cfg.freqs = fr.ctypes.data_as(_dp)
+    cfg.dft_stride = case.dft_stride
+    cfg.record_ports = int(case.record_ports)
+    cfg.periodic = case.periodic
+    cfg.flags = case.flags
+    cfg.precision = precision
+    return cfg, keep
+
+
+class Engine:
+    """One device-resident simulation (handle of the C ABI)."""
+
+    def __init__(self, case, precision=CFD_F64):
+        L = library()
+        cfg, self._keep = make_config(case, precision)
+        h = C.c_void_p()
+        _check(L.cfd_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.precision = precision
+        n = np.zeros(4, np.int32)
+        d = np.zeros(4, np.float64)
+        cnt = np.zeros(4, np.int32)
+        _check(L.cfd_info(self.h, n.ctypes.data_as(_ip), d.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip)))
+        self.N = tuple(int(x) for x in n[:3])
+        self.nt = int(n[3])
+        self.d = tuple(float(x) for x in d[:3])
+        self.dt = float(d[3])
+        self.shape = tuple(x + 1 for x in self.N)
+        self.nports, self.n_dl, self.n_gain, self.n_tfsf = (int(x) for x in cnt)
+        self.nf = len(case.freqs)
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().cfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping -------------------------------------------------------
+    def step(self, n=1):
+        _check(library().cfd_step(self.h, n))
+
+    def sync(self):
+        _check(library().cfd_sync(self.h))
+
+    @property
+    def steps_done(self):
+        return _check(library().cfd_steps_done(self.h))
+
+    def device_bytes(self):
+        return int(library().cfd_device_bytes(self.h))
+
+    def module_step(self, n, tfsf=True):
+        """One step as the reference's main.cpp sequences its modules
+        (main.cpp:267-343), one device pass per module call."""
+        L = library()
+        for f in (L.cfd_basic_h,):
+            _check(f(self.h))
+        if tfsf:
+            _check(L.cfd_tfsf_h(self.h, n))
+        _check(L.cfd_cpml_h(self.h))
+        _check(L.cfd_gain_pol(self.h))
+        _check(L.cfd_basic_e(self.h))
+        _check(L.cfd_dl_e(self.h))
+        _check(L.cfd_gain_e(self.h))
+        _check(L.cfd_gain_pop(self.h))
+        if tfsf:
+            _check(L.cfd_tfsf_e(self.h, n))
+        _check(L.cfd_sources(self.h, n))
+        _check(L.cfd_cpml_e(self.h))
+        if self.nports:
+            _check(L.cfd_ports_sample(self.h, n))
+        _check(L.cfd_advance(self.h))
+
+    # -- host access ----------------------------------------------------
+    def field(self, comp):
+        a = np.empty(self.shape, np.float64)
+        _check(library().cfd_read_field(self.h, comp, a.ctypes.data_as(_dp)))
+        return a
+
+    def fields(self):
+        return [self.field(c) for c in range(6)]
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
+        _check(library().cfd_write_field(self.h, comp, a.ctypes.data_as(_dp)))
+
+    def id_map(self):
+        a = np.empty(self.shape, np.int16)
+        _check(library().cfd_read_id(self.h, a.ctypes.data_as(C.POINTER(C.c_int16))))
+        return a
+
+    def port_points(self, q):
+        n = _check(library().cfd_port_npts(self.h, q))
+        a = np.empty((n, 3), np.int32)
+        _check(library().cfd_port_points(self.h, q, a.ctypes.data_as(_ip)))
+        return a
+
+    def port_samples(self, q):
+        n = _check(library().cfd_port_npts(self.h, q))
+        a = np.zeros((self.steps_done, n, 6), np.float64)
+        _check(library().cfd_port_samples(self.h, q, a.ctypes.data_as(_dp)))
+        return a
+
+    def port_dft(self, q):
+        n = _check(library().cfd_port_npts(self.h, q))
+        a = np.zeros((n, self.nf, 6, 2), np.float64)
+        _check(library().cfd_port_dft(self.h, q, a.ctypes.data_as(_dp)))
+        return a[..., 0] + 1j * a[..., 1]
+
+    def port_intensity(self, q):
+        a = np.zeros(self.steps_done, np.float64)
+        _check(library().cfd_port_intensity(self.h, q, a.ctypes.data_as(_dp)))
+        return a
+
+    def gain_pop(self):
+        a = np.zeros((self.n_gain, 4), np.float64)
+        _check(library().cfd_gain_read(self.h, a.ctypes.data_as(_dp)))
+        return a
+
+    # -- timing ---------------------------------------------------------
+    def set_timing(self, on=True):
+        _check(library().cfd_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        cap = 64
+        names = (C.c_char * 32 * cap)()
+        ms = np.zeros(cap, np.float64)
+        cnt = np.zeros(cap, np.int32)
+        n = _check(library().cfd_kernel_times(self.h, C.cast(names, C.c_void_p), ms.ctypes.data_as(_dp),
+                                              cnt.ctypes.data_as(_ip), cap))
+        return {bytes(names[i]).split(b"\0")[0].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+
+class Model:
+    """Host-side set-up of a case (no device): the tables the engine uploads."""
+
+    def __init__(self, case, precision=CFD_F64):
+        L = library()
+        cfg, keep = make_config(case, precision)
+        h = C.c_void_p()
+        if L.cfd_model_build(C.byref(cfg), C.byref(h)) < 0:
+            raise EngineError(L.cfd_model_last_error().decode())
+        self.h = h
+        n = np.zeros(4, np.int32)
+        d = np.zeros(4, np.float64)
+        L.cfd_model_dims(self.h, n.ctypes.data_as(_ip), d.ctypes.data_as(_dp))
+        self.N = tuple(int(x) for x in n[:3])
+        self.nt = int(n[3])
+        self.d = tuple(float(x) for x in d[:3])
+        self.dt = float(d[3])
+        self.shape = tuple(x + 1 for x in self.N)
+
+    def __del__(self):
+        try:
+            if self.h:
+                library().cfd_model_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def id_map(self):
+        a = np.empty(self.shape, np.int16)
+        library().cfd_model_id(self.h, a.ctypes.data_as(C.POINTER(C.c_int16)))
+        return a
+
+    def table(self, name):
+        L = library()
+        n = L.cfd_model_table(self.h, name.encode(), None)
+        if n < 0:
+            raise EngineError("no table " + name)
+        a = np.empty(n, np.float64)
+        L.cfd_model_table(self.h, name.encode(), a.ctypes.data_as(_dp))
+        return a
+
+    def tfsf(self, h):
+        L = library()
+        nterm = C.c_int(0)
+        nt = L.cfd_model_tfsf(self.h, h, None, None, C.byref(nterm))
+        tg = np.empty((max(nt, 1), 6), np.int32)
+        tm = np.empty((max(nterm.value, 1), 3), np.float64)
+        L.cfd_model_tfsf(self.h, h, tg.ctypes.data_as(_ip), tm.ctypes.data_as(_dp), C.byref(nterm))
+        return tg[:nt], tm[:nterm.value]
+
+    def port_points(self, q):
+        L = library()
+        n = L.cfd_model_port(self.h, q, None)
+        a = np.empty((max(n, 0), 3), np.int32)
+        L.cfd_model_port(self.h, q, a.ctypes.data_as(_ip))
+        return a
+
+    def cells(self, gain=False):
+        L = library()
+        n = L.cfd_model_cells(self.h, 1 if gain else 0, None)
+        a = np.empty(max(n, 1), np.int32)
+        L.cfd_model_cells(self.h, 1 if gain else 0, a.ctypes.data_as(_ip))
+        return a[:n]

@@ -1,0 +1,587 @@
+// engine.hip -- device state of one simulation, the step sequence, and the C
+// ABI declared in include/cpsfdtd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cpsfdtd.h"
+#include "engine.hpp"
+#include "kernels.hip"
+#include "model.hpp"
+
+namespace cfd {
+
+static thread_local std::string g_error;
+
+#define HIP_OK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) throw ::cfd::Error(std::string(#x) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+// ----------------------------------------------------------------------------
+// per-kernel timing with HIP events on the simulation's stream
+// ----------------------------------------------------------------------------
+struct KernelTimer {
+    bool on = false;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, int>> acc;
+
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        return e;
+    }
+    template <class F>
+    void run(hipStream_t s, const char *name, F &&launch) {
+        if (!on) { launch(); return; }
+        Pending p{name, get(), get()};
+        HIP_OK(hipEventRecord(p.a, s));
+        launch();
+        HIP_OK(hipEventRecord(p.b, s));
+        pending.push_back(p);
+    }
+    void collect() {
+        for (auto &p : pending) {
+            HIP_OK(hipEventSynchronize(p.b));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, p.a, p.b));
+            auto &a = acc[p.name];
+            a.first += ms;
+            a.second += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~KernelTimer() {
+        for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// ----------------------------------------------------------------------------
+class SimBase {
+public:
+    virtual ~SimBase() = default;
+    Model m;
+    int precision = 64;
+    hipStream_t stream = nullptr;
+    int done = 0;
+    size_t dev_bytes = 0;
+    KernelTimer timer;
+
+    virtual void step(int count) = 0;
+    virtual void basic_h() = 0;
+    virtual void basic_e() = 0;
+    virtual void cpml_h() = 0;
+    virtual void cpml_e() = 0;
+    virtual void tfsf(bool h, int n) = 0;
+    virtual void e_parts(unsigned parts) = 0;
+    virtual void sources(int n) = 0;
+    virtual void ports(int n) = 0;
+    virtual void read_field(int comp, double *host) = 0;
+    virtual void write_field(int comp, const double *host) = 0;
+    virtual void port_samples(int q, double *host) = 0;
+    virtual void port_dft(int q, double *host) = 0;
+    virtual void port_intensity(int q, double *host) = 0;
+    virtual void gain_read(double *host) = 0;
+};
+
+template <class R>
+class Sim : public SimBase {
+public:
+    Dev<R> d{};
+    std::vector<void *> allocs;
+    // TF/SF
+    TfTarget *tfE = nullptr, *tfH = nullptr;
+    int ntfE = 0, ntfH = 0, tablen = 0;
+    double *tcoef = nullptr, *tdelay = nullptr, *tprof = nullptr, *einc = nullptr, *hinc = nullptr;
+    // sources
+    long long *scell = nullptr;
+    int *scomp = nullptr;
+    double *swave = nullptr;
+    // ports
+    struct DevPort { long long *cell; int npts, nblk; R *samples; double *dft, *ipart; };
+    std::vector<DevPort> dports;
+    double *twE = nullptr, *twH = nullptr;
+
+    template <class T>
+    T *alloc(size_t count) {
+        if (!count) count = 1;
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, count * sizeof(T)));
+        HIP_OK(hipMemsetAsync(p, 0, count * sizeof(T), stream));
+        allocs.push_back(p);
+        dev_bytes += count * sizeof(T);
+        return static_cast<T *>(p);
+    }
+    template <class T, class S>
+    T *upload(const std::vector<S> &v) {
+        std::vector<T> tmp(v.begin(), v.end());
+        T *p = alloc<T>(tmp.size());
+        if (!tmp.empty()) HIP_OK(hipMemcpyAsync(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return p;
+    }
+    long long dev_index(int i, int j, int k) const {
+        return (long long)i + (long long)d.px * ((long long)j + (long long)d.py * k);
+    }
+
+    explicit Sim(Model &&model, int prec) {
+        m = std::move(model);
+        precision = prec;
+        HIP_OK(hipStreamCreate(&stream));
+        try {
+            setup();
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~Sim() override { release(); }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    void setup() {
+        d.Nx = m.Nx; d.Ny = m.Ny; d.Nz = m.Nz;
+        d.px = ((m.Nx + 1) + 31) / 32 * 32;
+        d.py = m.Ny + 1;
+        d.slab = (long long)d.px * d.py;
+        d.pbx = m.pbx; d.pby = m.pby;
+        d.ie0 = m.pbx ? 0 : 1;
+        d.je0 = m.pby ? 0 : 1;
+        const size_t ncell = (size_t)d.slab * (m.Nz + 1);
+        for (int c = 0; c < 6; c++) d.F[c] = alloc<R>(ncell);
+        // material IDs in device order
+        {
+            std::vector<uint8_t> id(ncell, 0);
+            for (int i = 0; i <= m.Nx; i++)
+                for (int j = 0; j <= m.Ny; j++)
+                    for (int k = 0; k <= m.Nz; k++) id[dev_index(i, j, k)] = (uint8_t)m.id[m.flat(i, j, k)];
+            d.id = upload<uint8_t>(id);
+        }
+        d.aE = upload<R>(m.aE); d.bE = upload<R>(m.bE); d.aH = upload<R>(m.aH); d.bH = upload<R>(m.bH);
+        for (int t = 0; t < kNumStretch; t++) d.st[t] = upload<R>(m.stretch[t]);
+        // CPML
+        d.has_pml = m.has_pml;
+        d.cpmlE = (R)m.cpmlE;
+        d.cpmlH = (R)m.cpmlH;
+        d.hsign_fixed = (m.flags & CFD_FIX_CPML_HSIGN) ? 1 : 0;
+        const int N[3] = {m.Nx, m.Ny, m.Nz};
+        for (int a = 0; a < 3; a++)
+            for (int e = 0; e < 2; e++) {
+                const int n = m.npml[a][e];
+                d.npml[a][e] = n;
+                if (!n) continue;
+                d.pb[0][a][e] = upload<R>(m.pbE[a][e]); d.pc[0][a][e] = upload<R>(m.pcE[a][e]);
+                d.pb[1][a][e] = upload<R>(m.pbH[a][e]); d.pc[1][a][e] = upload<R>(m.pcH[a][e]);
+                size_t box = a == 0 ? (size_t)n * d.py * (N[2] + 1) : a == 1 ? (size_t)d.px * n * (N[2] + 1)
+                                                                            : (size_t)d.slab * n;
+                for (int h = 0; h < 2; h++)
+                    for (int w = 0; w < 2; w++) d.q[h][a][e][w] = alloc<R>(box);
+            }
+        // Drude-Lorentz and gain cells: aux slot map
+        if (m.has_dl || m.has_gain) {
+            std::vector<int> aux(ncell, -1);
+            for (size_t s = 0; s < m.dl_cells.size(); s++) {
+                const size_t f = (size_t)m.dl_cells[s];
+                const int i = (int)(f / ((size_t)(m.Ny + 1) * (m.Nz + 1))), j = (int)((f / (m.Nz + 1)) % (m.Ny + 1)), k = (int)(f % (m.Nz + 1));
+                aux[dev_index(i, j, k)] = (int)s;
+            }
+            for (size_t s = 0; s < m.gain_cells.size(); s++) {
+                const size_t f = (size_t)m.gain_cells[s];
+                const int i = (int)(f / ((size_t)(m.Ny + 1) * (m.Nz + 1))), j = (int)((f / (m.Nz + 1)) % (m.Ny + 1)), k = (int)(f % (m.Nz + 1));
+                aux[dev_index(i, j, k)] = (int)s;
+            }
+            d.aux = upload<int>(aux);
+        }
+        d.dt = (R)m.dt;
+        if (m.has_dl) {
+            d.nD = m.nD; d.nL = m.nL; d.ndl = (int)m.dl_cells.size();
+            d.C1 = (R)m.dlC1; d.C2 = (R)m.dlC2; d.C3 = (R)m.dlC3;
+            for (int q = 0; q < m.nD; q++) { d.C4[q] = (R)m.dlC4[q]; d.kap[q] = (R)m.dlKap[q]; d.beta[q] = (R)m.dlBeta[q]; }
+            for (int q = 0; q < m.nL; q++) {
+                d.C5[q] = (R)m.dlC5[q]; d.C6[q] = (R)m.dlC6[q]; d.alpha[q] = (R)m.dlAlpha[q]; d.xi[q] = (R)m.dlXi[q]; d.gam[q] = (R)m.dlGam[q];
+            }
+            d.dlst = alloc<R>((size_t)(m.nD + 2 * m.nL + 1) * 3 * std::max<size_t>(d.ndl, 1));
+        }
+        if (m.has_gain) {
+            d.ngain = (int)m.gain_cells.size();
+            d.CPa1 = (R)m.gCPa1; d.CPa2 = (R)m.gCPa2; d.CPa3 = (R)m.gCPa3;
+            d.g_dt = m.dt;
+            d.g_hw = kHPlanck / (2.0 * kPi) * m.g_omega;
+            d.g_Ntot = m.g_Ntot;
+            d.g_h32 = m.dt / (2.0 * m.g_t32);
+            d.g_h21 = m.dt / (2.0 * m.g_t21);
+            d.g_h10 = m.dt / (2.0 * m.g_t10);
+            d.g_dtWp = m.dt * m.g_Wp;
+            const size_t ng = std::max<size_t>(d.ngain, 1);
+            d.gP = alloc<R>(3 * ng); d.gP1 = alloc<R>(3 * ng); d.gEprev = alloc<R>(3 * ng);
+            std::vector<double> N(4 * ng, 0.0);
+            for (size_t s = 0; s < (size_t)d.ngain; s++) N[s] = m.g_Ntot;
+            d.gN = upload<double>(N);
+        }
+        // TF/SF
+        if (m.has_tfsf) {
+            std::vector<TfTarget> te, th;
+            for (const TfsfTarget &t : m.tfE) te.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
+            for (const TfsfTarget &t : m.tfH) th.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
+            ntfE = (int)te.size(); ntfH = (int)th.size();
+            tfE = alloc<TfTarget>(te.size());
+            tfH = alloc<TfTarget>(th.size());
+            HIP_OK(hipMemcpyAsync(tfE, te.data(), te.size() * sizeof(TfTarget), hipMemcpyHostToDevice, stream));
+            HIP_OK(hipMemcpyAsync(tfH, th.data(), th.size() * sizeof(TfTarget), hipMemcpyHostToDevice, stream));
+            std::vector<double> c, dl, pr;
+            for (const TfsfTerm &t : m.tf_terms) { c.push_back(t.coef); dl.push_back(t.delay); pr.push_back(t.prof); }
+            tcoef = upload<double>(c); tdelay = upload<double>(dl); tprof = upload<double>(pr);
+            einc = upload<double>(m.Einc); hinc = upload<double>(m.Hinc);
+            tablen = (int)m.Einc.size();
+        }
+        // sources
+        if (!m.src.empty()) {
+            std::vector<long long> cells;
+            std::vector<int> comps;
+            for (size_t q = 0; q < m.src.size(); q += 4) {
+                cells.push_back(dev_index(m.src[q], m.src[q + 1], m.src[q + 2]));
+                comps.push_back(m.src[q + 3]);
+            }
+            scell = upload<long long>(cells); scomp = upload<int>(comps); swave = upload<double>(m.src_wave);
+        }
+        // ports
+        if (!m.freqs.empty()) { twE = upload<double>(m.twE); twH = upload<double>(m.twH); }
+        for (const Port &P : m.ports) {
+            DevPort dp{};
+            std::vector<long long> cells;
+            for (size_t q = 0; q < P.ijk.size(); q += 3) cells.push_back(dev_index(P.ijk[q], P.ijk[q + 1], P.ijk[q + 2]));
+            dp.npts = P.npts();
+            dp.nblk = (dp.npts + 255) / 256;
+            dp.cell = upload<long long>(cells);
+            dp.samples = m.record ? alloc<R>((size_t)m.nt * dp.npts * 6) : nullptr;
+            dp.dft = m.freqs.empty() ? nullptr : alloc<double>((size_t)dp.npts * m.freqs.size() * 12);
+            dp.ipart = alloc<double>((size_t)m.nt * std::max(dp.nblk, 1));
+            dports.push_back(dp);
+        }
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    dim3 grid() const { return dim3((m.Nx + TX - 1) / TX, (m.Ny + TY - 1) / TY, m.Nz); }
+
+    void launch_h(unsigned parts, const char *name) {
+        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_h<R>, grid(), dim3(TX, TY), 0, stream, d, parts); });
+        HIP_OK(hipGetLastError());
+    }
+    void launch_e(unsigned parts, const char *name) {
+        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_e<R>, grid(), dim3(TX, TY), 0, stream, d, parts); });
+        HIP_OK(hipGetLastError());
+    }
+    void tfsf(bool h, int n) override {
+        if (!m.has_tfsf) return;
+        const int nt = h ? ntfH : ntfE;
+        if (!nt) return;
+        timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
+            hipLaunchKernelGGL(k_tfsf<R>, dim3((nt + 255) / 256), dim3(256), 0, stream, d, h ? tfH : tfE, nt, tcoef,
+                               tdelay, tprof, h ? einc : hinc, tablen, n);
+        });
+        HIP_OK(hipGetLastError());
+    }
+    void sources(int n) override {
+        const int ns = (int)(m.src.size() / 4);
+        if (!ns) return;
+        if (n < 0 || n > m.nt) throw Error("source time index out of range");
+        timer.run(stream, "sources", [&] {
+            hipLaunchKernelGGL(k_sources<R>, dim3((ns + 63) / 64), dim3(64), 0, stream, d, scell, scomp, swave, ns, m.nt + 1, n);
+        });
+        HIP_OK(hipGetLastError());
+    }
+    void ports(int n) override {
+        if (n < 0 || n >= m.nt) throw Error("port time index out of range");
+        const int nf = (int)m.freqs.size();
+        const int do_dft = nf > 0 && (n % m.dft_stride == 0);
+        for (DevPort &p : dports) {
+            timer.run(stream, "ports", [&] {
+                hipLaunchKernelGGL(k_port<R>, dim3(p.nblk), dim3(256), 0, stream, d, p.cell, p.npts, n, p.samples,
+                                   p.dft, twE, twH, nf, m.nt, do_dft, p.ipart);
+            });
+            HIP_OK(hipGetLastError());
+        }
+    }
+    void basic_h() override { launch_h(H_BASIC, "basic_h"); }
+    void cpml_h() override { if (m.has_pml) launch_h(H_PML, "cpml_h"); }
+    void basic_e() override { launch_e(E_BASIC, "basic_e"); }
+    void cpml_e() override { if (m.has_pml) launch_e(E_PML, "cpml_e"); }
+    void e_parts(unsigned parts) override { launch_e(parts, "e_module"); }
+
+    // main.cpp:267-343: one fused H pass (basic + CPML), TF/SF-H, one fused E
+    // pass (gain polarisation, basic, Drude-Lorentz, gain current and
+    // populations, CPML), TF/SF-E, sources, ports
+    void step(int count) override {
+        for (int c = 0; c < count; c++) {
+            const int n = done;
+            if (n >= m.nt) throw Error("run finished: all " + std::to_string(m.nt) + " steps done");
+            launch_h(H_ALL, "update_h");
+            tfsf(true, n);
+            launch_e(E_ALL, "update_e");
+            tfsf(false, n);
+            sources(n);
+            if (!dports.empty()) ports(n);
+            done++;
+        }
+    }
+
+    void read_field(int comp, double *host) override {
+        const size_t ncell = (size_t)d.slab * (m.Nz + 1);
+        std::vector<R> tmp(ncell);
+        HIP_OK(hipMemcpyAsync(tmp.data(), d.F[comp], ncell * sizeof(R), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (int i = 0; i <= m.Nx; i++)
+            for (int j = 0; j <= m.Ny; j++)
+                for (int k = 0; k <= m.Nz; k++) host[m.flat(i, j, k)] = (double)tmp[dev_index(i, j, k)];
+    }
+    void write_field(int comp, const double *host) override {
+        const size_t ncell = (size_t)d.slab * (m.Nz + 1);
+        std::vector<R> tmp(ncell, (R)0);
+        for (int i = 0; i <= m.Nx; i++)
+            for (int j = 0; j <= m.Ny; j++)
+                for (int k = 0; k <= m.Nz; k++) tmp[dev_index(i, j, k)] = (R)host[m.flat(i, j, k)];
+        HIP_OK(hipMemcpyAsync(d.F[comp], tmp.data(), ncell * sizeof(R), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    void port_samples(int q, double *host) override {
+        const DevPort &p = dports.at(q);
+        if (!p.samples) throw Error("port samples were not recorded (record_ports = 0)");
+        const size_t n = (size_t)done * p.npts * 6;
+        std::vector<R> tmp(n);
+        if (n) HIP_OK(hipMemcpyAsync(tmp.data(), p.samples, n * sizeof(R), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (size_t t = 0; t < n; t++) host[t] = (double)tmp[t];
+    }
+    void port_dft(int q, double *host) override {
+        const DevPort &p = dports.at(q);
+        if (!p.dft) throw Error("no DFT frequencies configured");
+        const size_t nf = m.freqs.size(), np_ = p.npts;
+        std::vector<double> tmp(np_ * nf * 12);
+        HIP_OK(hipMemcpyAsync(tmp.data(), p.dft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (size_t f = 0; f < nf; f++)  // device [f][c][re,im][point] -> host [point][f][c][re,im]
+            for (size_t w = 0; w < 12; w++)
+                for (size_t t = 0; t < np_; t++) host[(t * nf + f) * 12 + w] = tmp[(f * 12 + w) * np_ + t];
+    }
+    void port_intensity(int q, double *host) override {
+        const DevPort &p = dports.at(q);
+        std::vector<double> tmp((size_t)done * p.nblk);
+        if (!tmp.empty()) HIP_OK(hipMemcpyAsync(tmp.data(), p.ipart, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (int n = 0; n < done; n++) {
+            double s = 0.0;
+            for (int b = 0; b < p.nblk; b++) s = s + tmp[(size_t)n * p.nblk + b];
+            host[n] = s;
+        }
+    }
+    void gain_read(double *host) override {
+        if (!m.has_gain) throw Error("no gain medium");
+        const size_t ng = d.ngain;
+        std::vector<double> tmp(4 * std::max<size_t>(ng, 1));
+        HIP_OK(hipMemcpyAsync(tmp.data(), d.gN, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (size_t s = 0; s < ng; s++)
+            for (int l = 0; l < 4; l++) host[s * 4 + l] = tmp[l * ng + s];
+    }
+};
+
+}  // namespace cfd
+
+// ============================================================================
+// C ABI
+// ============================================================================
+using cfd::SimBase;
+
+struct cfd_sim {
+    std::unique_ptr<SimBase> impl;
+};
+
+template <class F>
+static int guard(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        cfd::g_error = e.what();
+    } catch (...) {
+        cfd::g_error = "unknown error";
+    }
+    return -1;
+}
+
+#define NEED(s)                                          \
+    do {                                                 \
+        if (!(s) || !(s)->impl) {                        \
+            cfd::g_error = "null simulation handle";     \
+            return -1;                                   \
+        }                                                \
+    } while (0)
+
+extern "C" {
+
+const char *cfd_last_error(void) { return cfd::g_error.c_str(); }
+
+int cfd_device_info(char *name, int cap, int *n_cu, size_t *hbm_bytes)
+{
+    return guard([&] {
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        HIP_OK(hipGetDeviceProperties(&p, dev));
+        if (name && cap > 0) snprintf(name, cap, "%s (%s)", p.name, p.gcnArchName);
+        if (n_cu) *n_cu = p.multiProcessorCount;
+        if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+    });
+}
+
+int cfd_create(const cfd_config *cfg, cfd_sim **out)
+{
+    if (!cfg || !out) { cfd::g_error = "null argument"; return -1; }
+    *out = nullptr;
+    return guard([&] {
+        if (cfg->precision != CFD_F32 && cfg->precision != CFD_F64) throw cfd::Error("precision must be 32 or 64");
+        cfd::Model m = cfd::build_model(*cfg);
+        auto s = std::make_unique<cfd_sim>();
+        if (cfg->precision == CFD_F32) s->impl = std::make_unique<cfd::Sim<float>>(std::move(m), 32);
+        else s->impl = std::make_unique<cfd::Sim<double>>(std::move(m), 64);
+        *out = s.release();
+    });
+}
+
+int cfd_destroy(cfd_sim *s)
+{
+    if (!s) return 0;
+    delete s;
+    return 0;
+}
+
+int cfd_info(cfd_sim *s, int *nxyz_nt, double *dd, int *counts)
+{
+    NEED(s);
+    const cfd::Model &m = s->impl->m;
+    if (nxyz_nt) { nxyz_nt[0] = m.Nx; nxyz_nt[1] = m.Ny; nxyz_nt[2] = m.Nz; nxyz_nt[3] = m.nt; }
+    if (dd) { dd[0] = m.dx; dd[1] = m.dy; dd[2] = m.dz; dd[3] = m.dt; }
+    if (counts) {
+        counts[0] = (int)m.ports.size();
+        counts[1] = (int)m.dl_cells.size();
+        counts[2] = (int)m.gain_cells.size();
+        counts[3] = (int)(m.tfE.size() + m.tfH.size());
+    }
+    return 0;
+}
+
+size_t cfd_device_bytes(const cfd_sim *s) { return (s && s->impl) ? s->impl->dev_bytes : 0; }
+
+int cfd_step(cfd_sim *s, int count) { NEED(s); return guard([&] { s->impl->step(count); }); }
+int cfd_steps_done(cfd_sim *s) { NEED(s); return s->impl->done; }
+int cfd_sync(cfd_sim *s)
+{
+    NEED(s);
+    return guard([&] {
+        HIP_OK(hipStreamSynchronize(s->impl->stream));
+        s->impl->timer.collect();
+    });
+}
+int cfd_basic_h(cfd_sim *s) { NEED(s); return guard([&] { s->impl->basic_h(); }); }
+int cfd_basic_e(cfd_sim *s) { NEED(s); return guard([&] { s->impl->basic_e(); }); }
+int cfd_cpml_h(cfd_sim *s) { NEED(s); return guard([&] { s->impl->cpml_h(); }); }
+int cfd_cpml_e(cfd_sim *s) { NEED(s); return guard([&] { s->impl->cpml_e(); }); }
+int cfd_tfsf_h(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->tfsf(true, n); }); }
+int cfd_tfsf_e(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->tfsf(false, n); }); }
+int cfd_dl_e(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_dl) s->impl->e_parts(cfd::E_DL); }); }
+int cfd_gain_pol(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOL); }); }
+int cfd_gain_e(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINE); }); }
+int cfd_gain_pop(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOP); }); }
+int cfd_sources(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->sources(n); }); }
+int cfd_ports_sample(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->ports(n); }); }
+int cfd_advance(cfd_sim *s) { NEED(s); s->impl->done++; return 0; }
+
+int cfd_read_field(cfd_sim *s, int comp, double *host)
+{
+    NEED(s);
+    if (comp < 0 || comp > 5 || !host) { cfd::g_error = "bad component or buffer"; return -1; }
+    return guard([&] { s->impl->read_field(comp, host); });
+}
+int cfd_write_field(cfd_sim *s, int comp, const double *host)
+{
+    NEED(s);
+    if (comp < 0 || comp > 5 || !host) { cfd::g_error = "bad component or buffer"; return -1; }
+    return guard([&] { s->impl->write_field(comp, host); });
+}
+int cfd_read_id(cfd_sim *s, int16_t *host)
+{
+    NEED(s);
+    if (!host) { cfd::g_error = "null buffer"; return -1; }
+    std::memcpy(host, s->impl->m.id.data(), s->impl->m.id.size() * sizeof(int16_t));
+    return 0;
+}
+int cfd_port_npts(cfd_sim *s, int port)
+{
+    NEED(s);
+    if (port < 0 || port >= (int)s->impl->m.ports.size()) { cfd::g_error = "no such port"; return -1; }
+    return s->impl->m.ports[port].npts();
+}
+int cfd_port_points(cfd_sim *s, int port, int *ijk)
+{
+    NEED(s);
+    if (port < 0 || port >= (int)s->impl->m.ports.size() || !ijk) { cfd::g_error = "no such port"; return -1; }
+    const auto &v = s->impl->m.ports[port].ijk;
+    std::memcpy(ijk, v.data(), v.size() * sizeof(int));
+    return 0;
+}
+int cfd_port_samples(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_samples(port, host); }); }
+int cfd_port_dft(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_dft(port, host); }); }
+int cfd_port_intensity(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_intensity(port, host); }); }
+int cfd_gain_read(cfd_sim *s, double *host) { NEED(s); return guard([&] { s->impl->gain_read(host); }); }
+
+int cfd_set_timing(cfd_sim *s, int on)
+{
+    NEED(s);
+    return guard([&] {
+        HIP_OK(hipStreamSynchronize(s->impl->stream));
+        s->impl->timer.collect();
+        s->impl->timer.acc.clear();
+        s->impl->timer.on = on != 0;
+    });
+}
+
+int cfd_kernel_times(cfd_sim *s, char (*names)[32], double *ms, int *launches, int cap)
+{
+    NEED(s);
+    int n = 0;
+    int rc = guard([&] {
+        HIP_OK(hipStreamSynchronize(s->impl->stream));
+        s->impl->timer.collect();
+        for (auto &kv : s->impl->timer.acc) {
+            if (n >= cap) break;
+            snprintf(names[n], 32, "%s", kv.first.c_str());
+            ms[n] = kv.second.first;
+            launches[n] = kv.second.second;
+            n++;
+        }
+    });
+    return rc ? rc : n;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// engine.hpp -- device-side view of one simulation and the kernel interface.
+//
+// Data layout in HBM (all per simulation):
+//   fields Ex..Hz     structure of arrays, x fastest: p = i + px*(j + py*k),
+//                     px = Nx+1 rounded up to 32 elements (128-byte rows for
+//                     both FP32 and FP64), py = Ny+1; (Nx+1)..px-1 is padding
+//   material ID       uint8 in the same layout
+//   aux index         int32 in the same layout, slot of a Drude-Lorentz or gain
+//                     cell in its state arrays (only when those modules exist)
+//   CPML psi          slab boxes, x fastest; thickness n along the slab axis
+//   coefficient and stretch tables, TF/SF target lists, port point lists
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace cfd {
+
+constexpr int kMaxPoles = 8;
+constexpr int TX = 64;  // tile width along x: one wavefront
+constexpr int TY = 4;   // tile height along y: wavefronts per workgroup
+
+// parts of the E half-step a kernel performs (fused step = all of them)
+enum : unsigned {
+    E_BASIC = 1u,    // BasicFDTD.cpp:25-45
+    E_DL = 2u,       // dispersive cells (ID 2)
+    E_GAINPOL = 4u,  // gain polarisation P^{n+1} from E^n (ID 5)
+    E_GAINE = 8u,    // E -= (bE/dt) dP
+    E_GAINPOP = 16u, // four-level populations
+    E_PML = 32u,     // CPML.cpp:529-607
+    E_ALL = 63u,
+};
+enum : unsigned { H_BASIC = 1u, H_PML = 2u, H_ALL = 3u };
+
+template <class R>
+struct Dev {
+    int Nx, Ny, Nz, px, py;
+    long long slab;  // px*py
+    int pbx, pby, ie0, je0;
+    R *F[6];
+    const uint8_t *id;
+    const int *aux;
+    const R *aE, *bE, *aH, *bH;
+    const R *st[12];
+    // CPML
+    int has_pml;
+    int npml[3][2];
+    const R *pb[2][3][2], *pc[2][3][2];  // [E/H][axis][side]
+    R *q[2][3][2][2];                    // [E/H][axis][side][which]
+    R cpmlE, cpmlH;
+    int hsign_fixed;
+    // Drude-Lorentz
+    int nD, nL, ndl;
+    R C1, C2, C3, dt;
+    R C4[kMaxPoles], C5[kMaxPoles], C6[kMaxPoles], kap[kMaxPoles], beta[kMaxPoles];
+    R alpha[kMaxPoles], xi[kMaxPoles], gam[kMaxPoles];
+    R *dlst;  // [(var*3 + comp) * ndl + slot]
+    // gain (populations FP64)
+    int ngain;
+    R CPa1, CPa2, CPa3;
+    double g_dt, g_hw, g_Ntot, g_h32, g_h21, g_h10, g_dtWp;
+    R *gP, *gP1, *gEprev;  // [comp * ngain + slot]
+    double *gN;            // [level * ngain + slot], N0..N3
+};
+
+}  // namespace cfd

@@ -1,0 +1,139 @@
+/*
+ * cpsfdtd.h -- C ABI of the MI355X FDTD engine (libcpsfdtd.so).
+ *
+ * One opaque handle per simulation; its state (fields, CPML psi, dispersive
+ * and gain auxiliaries, port accumulators) stays resident in HBM.  Every call
+ * on a handle is ordered on the handle's HIP stream; calls that return host
+ * data synchronise that stream.  Return value 0 = success, negative = error,
+ * cfd_last_error() then holds a message (thread-local).
+ *
+ * Field arrays crossing this boundary are FP64 in the reference's order:
+ * (Nx+1)(Ny+1)(Nz+1) values, [i][j][k], k fastest (Fields.cpp:99-131).
+ *
+ * The reference has no FFI; the entry points replace its C++ class calls in
+ * main.cpp's time loop, cited per function.
+ */
+#ifndef CPSFDTD_H
+#define CPSFDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* behaviour switches (bit values shared with the oracle) */
+#define CFD_FIX_CPML_HSIGN   1u  /* CPML.cpp:643-682 H-side y/z psi sign -> Roden-Gedney */
+#define CFD_FIX_TFSF_FARFACE 2u  /* TF_SF.cpp:1222,1255,1288 far-face H at index+0 */
+#define CFD_SRC_REF_PULSE    4u  /* point sources follow Sources.cpp:72-76 Pulse() */
+#define CFD_REF_ZEROHEAP     8u  /* reproduce the reference's reads of zero-filled unset memory */
+
+#define CFD_PORT_PLANE  0
+#define CFD_PORT_SPHERE 1
+
+#define CFD_F32 32
+#define CFD_F64 64
+
+typedef struct {
+    const char *material;   /* Structure(const char*)              Structure.cpp:26   */
+    const char *structure;  /* Structure::generate_mesh            Structure.cpp:68   */
+    const char *pml;        /* CPML / CFS_PML(Structure&, char*)   CPML.cpp:29        NULL = none */
+    const char *tfsf;       /* TF_SF(Structure&, char*)            TF_SF.cpp:29       NULL = none */
+    const char *dl;         /* Drude_Lorentz(Structure&, char*)    Drude_Lorentz.cpp:30 (3-D layout) */
+    const char *gain;       /* FourLevelLaser(Structure&, char*)   FourLevelLaser.cpp:30 (+ pump line) */
+    const char *source;     /* Source(double, char*)               Sources.cpp:29     */
+    int n_ports;
+    const char *const *port_files;   /* SphericalPort / PlainPort(Structure&, const char*) Ports.cpp:28,201 */
+    const int *port_kinds;           /* CFD_PORT_PLANE / CFD_PORT_SPHERE */
+    int n_src;                       /* soft point sources (main.cpp:284-288) */
+    const int *src_ijkc;             /* i j k comp(0..2) per source */
+    const double *src_amp;
+    const double *src_phase;
+    int nt;                          /* steps; <= 0 takes Nt from TF_SF.txt (main.cpp:267) */
+    int n_freq;                      /* running-DFT frequencies at the ports (Hz) */
+    const double *freqs;
+    int dft_stride;                  /* accumulate every dft_stride-th step */
+    int record_ports;                /* keep every step's port samples on the device */
+    int periodic;                    /* bit0 x, bit1 y periodic (PBC.h:21-38 stub, built new) */
+    unsigned flags;                  /* CFD_* switches */
+    int precision;                   /* CFD_F32 or CFD_F64 field storage and update */
+} cfd_config;
+
+typedef struct cfd_sim cfd_sim;
+
+const char *cfd_last_error(void);
+int cfd_device_info(char *name, int cap, int *n_cu, size_t *hbm_bytes);
+
+/* Structure + Field + CPML::InitializeCPML + TF_SF + Drude_Lorentz +
+ * FourLevelLaser + ports construction (main.cpp:115-147), then upload. */
+int cfd_create(const cfd_config *cfg, cfd_sim **out);
+int cfd_destroy(cfd_sim *s);
+/* nxyz_nt: Nx Ny Nz nt; d: dx dy dz dt; counts: n_ports, n_dl_cells, n_gain_cells, n_tfsf_targets */
+int cfd_info(cfd_sim *s, int *nxyz_nt, double *d, int *counts);
+size_t cfd_device_bytes(const cfd_sim *s);
+
+/* One full time step per count, fused kernels (main.cpp:267-343 order:
+ * H, TF/SF-H, CPML-H, E, DL, gain, TF/SF-E, sources, CPML-E, ports). */
+int cfd_step(cfd_sim *s, int count);
+int cfd_steps_done(cfd_sim *s);
+int cfd_sync(cfd_sim *s);
+
+/* Single-module passes in the reference's granularity, for drivers that keep
+ * main.cpp's call sequence.  n is the time index the reference passes. */
+int cfd_basic_h(cfd_sim *s);            /* BasicHFieldUpdate      BasicFDTD.cpp:67-81   */
+int cfd_basic_e(cfd_sim *s);            /* BasicEFieldUpdate      BasicFDTD.cpp:25-45   */
+int cfd_cpml_h(cfd_sim *s);             /* CPML::UpdateCPMLHFields CPML.cpp:609-689     */
+int cfd_cpml_e(cfd_sim *s);             /* CPML::UpdateCPMLEFields CPML.cpp:529-607     */
+int cfd_tfsf_h(cfd_sim *s, int n);      /* TF_SF::UpdateHField    TF_SF.cpp:1201-1301   */
+int cfd_tfsf_e(cfd_sim *s, int n);      /* TF_SF::UpdateEField    TF_SF.cpp:988-1089    */
+int cfd_dl_e(cfd_sim *s);               /* Drude_Lorentz::EFieldUpdate Drude_Lorentz.cpp:153-227 (3-D) */
+int cfd_gain_pol(cfd_sim *s);           /* FourLevelLaser::UpdatePolarizationDensities FourLevelLaser.cpp:270-298 */
+int cfd_gain_e(cfd_sim *s);             /* FourLevelLaser::UpdateElectricField FourLevelLaser.cpp:300-308 */
+int cfd_gain_pop(cfd_sim *s);           /* FourLevelLaser::UpdateStatesPopulations FourLevelLaser.cpp:310-326 */
+int cfd_sources(cfd_sim *s, int n);     /* soft point sources     main.cpp:284-288      */
+int cfd_ports_sample(cfd_sim *s, int n);/* *Port::Update_Fields_on_Port Ports.cpp:88-99,342-353 */
+int cfd_advance(cfd_sim *s);            /* count one step done after a module sequence */
+
+/* Host access (synchronising). */
+int cfd_read_field(cfd_sim *s, int comp, double *host);        /* comp 0..5 Ex Ey Ez Hx Hy Hz */
+int cfd_write_field(cfd_sim *s, int comp, const double *host);
+int cfd_read_id(cfd_sim *s, int16_t *host);                     /* Structure::ID */
+int cfd_port_npts(cfd_sim *s, int port);
+int cfd_port_points(cfd_sim *s, int port, int *ijk);
+int cfd_port_samples(cfd_sim *s, int port, double *host);       /* [steps][npts][6] */
+int cfd_port_dft(cfd_sim *s, int port, double *host);           /* [npts][n_freq][6][2] */
+int cfd_port_intensity(cfd_sim *s, int port, double *host);     /* [steps] sum of |E|^2 */
+int cfd_gain_read(cfd_sim *s, double *host);                    /* [cells][4] N0..N3 */
+
+/* Host-side set-up only, no device: the parsed model the engine uploads
+ * (Structure, CPML, TF_SF, port constructors).  Usable without a GPU. */
+typedef struct cfd_model cfd_model;
+int cfd_model_build(const cfd_config *cfg, cfd_model **out);
+int cfd_model_free(cfd_model *m);
+const char *cfd_model_last_error(void);
+int cfd_model_dims(cfd_model *m, int *nxyz_nt, double *d);
+int cfd_model_id(cfd_model *m, int16_t *host);                      /* Structure::ID */
+/* named FP64 table; returns its length (call with out = NULL to size).
+ * names: aE bE aH bH (Structure.cpp:352-357), stretch0..stretch11
+ * (cExy cEyx cEzx cExz cEzy cEyz cHxy cHyx cHzx cHxz cHzy cHyz after CPML
+ * kappa folding), pml_{E,H}{b,c}{x,y,z}{1,2} (CPML.cpp:139-229), Einc, Hinc
+ * (TF_SF.cpp:898-955), dl (C1 C2 C3 C4[] C5[] C6[] kappa[] beta[] alpha[] xi[] gamma[]) */
+int cfd_model_table(cfd_model *m, const char *name, double *out);
+/* TF/SF correction lists: targets as comp i j k first count (6 ints), terms
+ * as coef delay profile (3 doubles); returns the number of targets, *nterms
+ * receives the number of terms.  h = 0 for E corrections, 1 for H. */
+int cfd_model_tfsf(cfd_model *m, int h, int *targets, double *terms, int *nterms);
+int cfd_model_port(cfd_model *m, int port, int *ijk);               /* returns point count */
+/* DL / gain cell lists in reference [i][j][k] flat order; returns count */
+int cfd_model_cells(cfd_model *m, int gain, int *flat);
+
+/* Per-kernel timing of the last cfd_step call (HIP events on the sim's stream):
+ * names[i] (<=32 chars) and total milliseconds; returns entries. */
+int cfd_kernel_times(cfd_sim *s, char (*names)[32], double *ms, int *launches, int cap);
+int cfd_set_timing(cfd_sim *s, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
