@@ -31,25 +31,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(eng_case_model, S, nd_vars):
-    """Algorithmic HBM bytes per launch of the fused H and E kernels: each
-    updated cell reads its own 6 field values and ID byte and writes 3 field
-    values (neighbour values are reuse, not traffic); CPML slab cells read and
-    write 2 psi values per slab they lie in; Drude-Lorentz cells read their
-    aux slot (4 B) and read/write 3 x nd_vars auxiliaries."""
-    m = eng_case_model
-    Nx, Ny, Nz = m["N"]
-    n = m["npml"]
-    cells_h = Nx * Ny * Nz
-    cells_e = (Nx - 1) * (Ny - 1) * (Nz - 1)
+def pass_geometry(N, npml, periodic=0):
+    """Cell classes of the fused step (same rule as the engine's Sim::classify):
+    per half-step the lean interior box and the CPML shell boxes."""
+    out = {}
+    for ef in (False, True):
+        lo, a, b = [0] * 3, [0] * 3, [0] * 3
+        for ax in range(3):
+            n1, n2 = npml[ax]
+            lo[ax] = ((0 if periodic & (1 << ax) else 1) if ax < 2 else 1) if ef else 0
+            a[ax] = (n1 + 1 if ef else n1) if n1 else lo[ax]
+            b[ax] = N[ax] - n2
+        lean = (a[0], b[0], a[1], b[1], a[2], b[2])
+        total = (N[0] - lo[0]) * (N[1] - lo[1]) * (N[2] - lo[2])
+        nlean = max(0, b[0] - a[0]) * max(0, b[1] - a[1]) * max(0, b[2] - a[2])
+        slab = 0  # slab memberships (a corner cell counts once per slab)
+        for ax in range(3):
+            n1, n2 = npml[ax]
+            plane = 1
+            for o in range(3):
+                if o != ax:
+                    plane *= N[o] - lo[o]
+            slab += (n1 + n2) * plane
+        out["e" if ef else "h"] = {"lean_box": lean, "lean": nlean, "shell": total - nlean, "slab": slab}
+    return out
+
+
+def algorithmic_bytes(geo, S, nd_vars):
+    """Algorithmic HBM bytes per launch of each update pass: an updated cell
+    reads its own 6 field values and its ID byte and writes 3 field values
+    (neighbour values are reuse, not traffic); each slab a CPML cell lies in
+    adds 2 psi values read and written; a Drude-Lorentz cell adds its 4-byte
+    slot and 3 x nd_vars auxiliaries read and written (plus an 8-byte cell
+    index in the special pass); a lean-pass cell it skips costs its ID byte."""
     per_cell = 9 * S + 1
-    slab_h = (n[0][0] + n[0][1]) * Ny * Nz + Nx * (n[1][0] + n[1][1]) * Nz + Nx * Ny * (n[2][0] + n[2][1])
-    slab_e = (n[0][0] + n[0][1]) * (Ny - 1) * (Nz - 1) + (Nx - 1) * (n[1][0] + n[1][1]) * (Nz - 1) + \
-        (Nx - 1) * (Ny - 1) * (n[2][0] + n[2][1])
-    psi = 2 * 2 * S
-    bh = cells_h * per_cell + slab_h * psi
-    be = cells_e * per_cell + slab_e * psi + m["ndl"] * (4 + 3 * nd_vars * 2 * S)
-    return {"update_h": bh, "update_e": be, "cells_h": cells_h, "cells_e": cells_e, "slab_h": slab_h, "slab_e": slab_e}
+    dl_bytes = 4 + 3 * nd_vars * 2 * S
+    h, e = geo["h"], geo["e"]
+    nsp, nsh_dl = geo["dl_interior"], geo["dl_shell"]
+    return {
+        "lean_h": h["lean"] * per_cell,
+        "shell_h": h["shell"] * per_cell + h["slab"] * 4 * S,
+        "lean_e": (e["lean"] - nsp) * per_cell + nsp,
+        "shell_e": e["shell"] * per_cell + e["slab"] * 4 * S + nsh_dl * dl_bytes,
+        "special_e": nsp * (per_cell + 8 + dl_bytes),
+    }
 
 
 def run_engine(case, precision, steps, warmup, timing_steps):
@@ -107,7 +132,19 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        n = 1 << 28
+        a = torch.ones(n, device="cuda:0", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": n * 4, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (multi-GPU sharding is a design "
               "note only)" % args.gpus, file=sys.stderr)
@@ -124,9 +161,17 @@ def main():
     N = model.N
     cells = N[0] * N[1] * N[2]
     npml = [[len(model.table("pml_Hb%s%d" % (ax, e + 1))) for e in range(2)] for ax in "xyz"]
-    ndl = len(model.cells(False))
     nd_vars = 1 + 2 * 2 + 1  # 1 Drude + 2 Lorentz poles: JD, JL, JL1, E1
-    geo = {"N": N, "npml": npml, "ndl": ndl}
+    geo = pass_geometry(N, npml)
+    lb = geo["e"]["lean_box"]
+    dl_in = 0
+    cells_dl = model.cells(False)
+    for f in cells_dl:
+        f = int(f)
+        i, j, k = f // ((N[1] + 1) * (N[2] + 1)), (f // (N[2] + 1)) % (N[1] + 1), f % (N[2] + 1)
+        if lb[0] <= i < lb[1] and lb[2] <= j < lb[3] and lb[4] <= k < lb[5]:
+            dl_in += 1
+    geo["dl_interior"], geo["dl_shell"] = dl_in, len(cells_dl) - dl_in
     del model
 
     res = {}
@@ -138,14 +183,15 @@ def main():
         kstat = {}
         for name, (ms, cnt) in kt.items():
             kstat[name] = {"avg_ms": ms / cnt, "launches": cnt}
-        dom = max(("update_h", "update_e"), key=lambda k: kt.get(k, (0, 1))[0])
+        dom = max((k for k in ab if k in kt), key=lambda k: kt[k][0])
         avg_s = kstat[dom]["avg_ms"] / 1e3
         step_ms = sum(v["avg_ms"] * v["launches"] for v in kstat.values()) / args.timing_steps
         res[prec] = {
             "wall": wall, "mcells": cells * args.steps / wall / 1e6, "ms_per_step": wall / args.steps * 1e3,
             "kernels": kstat, "dominant": dom, "dom_gbs": ab[dom] / avg_s / 1e9, "dom_bytes": ab[dom],
-            "step_gbs": (ab["update_h"] + ab["update_e"]) / (wall / args.steps) / 1e9, "device_bytes": nbytes,
+            "step_gbs": sum(ab.values()) / (wall / args.steps) / 1e9, "device_bytes": nbytes,
             "kernel_ms_per_step": step_ms,
+            "pass_gbs": {k: round(ab[k] / (kstat[k]["avg_ms"] / 1e3) / 1e9, 1) for k in ab if k in kstat},
         }
         log("bench: FP%d %.1f Mcell-upd/s, %.3f ms/step, %s %.1f GB/s" % (
             prec, res[prec]["mcells"], res[prec]["ms_per_step"], dom, res[prec]["dom_gbs"]))
@@ -173,6 +219,7 @@ def main():
                      "avg_ms": round(r32["kernels"][r32["dominant"]]["avg_ms"], 5)},
         "kernels_f32": {k: round(v["avg_ms"], 5) for k, v in r32["kernels"].items()},
         "step_gbs_f32": round(r32["step_gbs"], 1),
+        "pass_gbs_f32": r32["pass_gbs"],
         "device": info["name"],
     }
     if 64 in res:

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -113,8 +114,12 @@ public:
     int *scomp = nullptr;
     double *swave = nullptr;
     // ports
-    struct DevPort { long long *cell; int npts, nblk; R *samples; double *dft, *ipart; };
-    std::vector<DevPort> dports;
+    std::vector<int> dl_slot, gain_slot;  // reference-order cell -> device slot
+    std::vector<PortDesc<R>> dports;  // host copy of the port table
+    PortDesc<R> *port_tab = nullptr;
+    int port_blocks = 0;
+    static constexpr int kDftBatch = 16;  // sampled steps buffered per DFT pass
+    int dft_pending = 0, dft_first = 0;
     double *twE = nullptr, *twH = nullptr;
 
     template <class T>
@@ -160,6 +165,7 @@ public:
     }
 
     void setup() {
+        if (const char *v = std::getenv("CFD_UPDATE_KERNEL")) split_passes = std::string(v) != "grid";
         d.Nx = m.Nx; d.Ny = m.Ny; d.Nz = m.Nz;
         d.px = ((m.Nx + 1) + 31) / 32 * 32;
         d.py = m.Ny + 1;
@@ -197,19 +203,28 @@ public:
                 for (int h = 0; h < 2; h++)
                     for (int w = 0; w < 2; w++) d.q[h][a][e][w] = alloc<R>(box);
             }
-        // Drude-Lorentz and gain cells: aux slot map
+        // Drude-Lorentz and gain cells: slots numbered in device order (x fastest),
+        // so the lanes of a wavefront touch consecutive state words
         if (m.has_dl || m.has_gain) {
             std::vector<int> aux(ncell, -1);
-            for (size_t s = 0; s < m.dl_cells.size(); s++) {
-                const size_t f = (size_t)m.dl_cells[s];
-                const int i = (int)(f / ((size_t)(m.Ny + 1) * (m.Nz + 1))), j = (int)((f / (m.Nz + 1)) % (m.Ny + 1)), k = (int)(f % (m.Nz + 1));
-                aux[dev_index(i, j, k)] = (int)s;
-            }
-            for (size_t s = 0; s < m.gain_cells.size(); s++) {
-                const size_t f = (size_t)m.gain_cells[s];
-                const int i = (int)(f / ((size_t)(m.Ny + 1) * (m.Nz + 1))), j = (int)((f / (m.Nz + 1)) % (m.Ny + 1)), k = (int)(f % (m.Nz + 1));
-                aux[dev_index(i, j, k)] = (int)s;
-            }
+            auto assign = [&](const std::vector<int> &cells, std::vector<int> &slot_of) {
+                std::vector<std::pair<long long, int>> order;
+                order.reserve(cells.size());
+                for (size_t r = 0; r < cells.size(); r++) {
+                    const size_t f = (size_t)cells[r];
+                    const int i = (int)(f / ((size_t)(m.Ny + 1) * (m.Nz + 1)));
+                    const int j = (int)((f / (m.Nz + 1)) % (m.Ny + 1)), k = (int)(f % (m.Nz + 1));
+                    order.emplace_back(dev_index(i, j, k), (int)r);
+                }
+                std::sort(order.begin(), order.end());
+                slot_of.assign(cells.size(), 0);
+                for (size_t s2 = 0; s2 < order.size(); s2++) {
+                    aux[order[s2].first] = (int)s2;
+                    slot_of[order[s2].second] = (int)s2;
+                }
+            };
+            assign(m.dl_cells, dl_slot);
+            assign(m.gain_cells, gain_slot);
             d.aux = upload<int>(aux);
         }
         d.dt = (R)m.dt;
@@ -267,28 +282,151 @@ public:
         // ports
         if (!m.freqs.empty()) { twE = upload<double>(m.twE); twH = upload<double>(m.twH); }
         for (const Port &P : m.ports) {
-            DevPort dp{};
+            PortDesc<R> dp{};
             std::vector<long long> cells;
             for (size_t q = 0; q < P.ijk.size(); q += 3) cells.push_back(dev_index(P.ijk[q], P.ijk[q + 1], P.ijk[q + 2]));
             dp.npts = P.npts();
             dp.nblk = (dp.npts + 255) / 256;
+            dp.bstart = port_blocks;
+            port_blocks += dp.nblk;
             dp.cell = upload<long long>(cells);
             dp.samples = m.record ? alloc<R>((size_t)m.nt * dp.npts * 6) : nullptr;
+            dp.batch = m.freqs.empty() ? nullptr : alloc<R>((size_t)kDftBatch * 6 * dp.npts);
             dp.dft = m.freqs.empty() ? nullptr : alloc<double>((size_t)dp.npts * m.freqs.size() * 12);
             dp.ipart = alloc<double>((size_t)m.nt * std::max(dp.nblk, 1));
             dports.push_back(dp);
         }
+        setup_passes();
+        if (!dports.empty()) {
+            port_tab = alloc<PortDesc<R>>(dports.size());
+            HIP_OK(hipMemcpyAsync(port_tab, dports.data(), dports.size() * sizeof(PortDesc<R>), hipMemcpyHostToDevice, stream));
+        }
         HIP_OK(hipStreamSynchronize(stream));
     }
 
-    dim3 grid() const { return dim3((m.Nx + TX - 1) / TX, (m.Ny + TY - 1) / TY, m.Nz); }
+    TileGrid tiles() const {
+        TileGrid g;
+        g.gx = (m.Nx + TX - 1) / TX;
+        g.gy = (m.Ny + TY - 1) / TY;
+        g.gz = m.Nz;
+        return g;
+    }
+    // Fused-step update passes: lean interior box, CPML shell boxes, special
+    // (Drude-Lorentz / gain) interior cells.  CFD_UPDATE_KERNEL=grid selects the
+    // single full-physics grid kernel per half-step instead (A/B comparison).
+    bool split_passes = true;
+    Box leanH{}, leanE{};
+    std::vector<Box> shellH, shellE;
+    Box *shellH_dev = nullptr, *shellE_dev = nullptr;
+    int shellH_blocks = 0, shellE_blocks = 0;
+    long long *special = nullptr;
+    int nspecial = 0;
+
+    static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1, int tw) {
+        Box b{};
+        b.i0 = i0; b.i1 = i1; b.j0 = j0; b.j1 = j1; b.k0 = k0; b.k1 = k1;
+        b.tw = tw; b.th = 256 / tw;
+        b.xbase = (i0 / tw) * tw;
+        b.gx = (i1 - b.xbase + tw - 1) / tw;
+        b.gy = (j1 - j0 + b.th - 1) / b.th;
+        return b;
+    }
+    static bool empty(const Box &b) { return b.i1 <= b.i0 || b.j1 <= b.j0 || b.k1 <= b.k0; }
+    static long long blocks(const Box &b) { return empty(b) ? 0 : (long long)b.gx * b.gy * (b.k1 - b.k0); }
+
+    // cell classes of one half-step: interior box and shell boxes.  lo/hi give the
+    // update range, a/b the first and last interior index per axis.
+    void classify(bool efield, Box &lean, std::vector<Box> &shell) {
+        const int N[3] = {m.Nx, m.Ny, m.Nz};
+        int lo[3], a[3], b[3];
+        for (int ax = 0; ax < 3; ax++) {
+            const int n1 = m.npml[ax][0], n2 = m.npml[ax][1];
+            lo[ax] = efield ? (ax == 0 ? d.ie0 : ax == 1 ? d.je0 : 1) : 0;
+            a[ax] = n1 ? (efield ? n1 + 1 : n1) : lo[ax];
+            b[ax] = N[ax] - n2;
+        }
+        lean = make_box(a[0], b[0], a[1], b[1], a[2], b[2], TX);
+        auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1) {
+            const int tw = (i1 - i0 <= 16) ? 16 : TX;
+            Box bx = make_box(i0, i1, j0, j1, k0, k1, tw);
+            if (!empty(bx)) shell.push_back(bx);
+        };
+        add(lo[0], N[0], lo[1], N[1], lo[2], a[2]);       // z low
+        add(lo[0], N[0], lo[1], N[1], b[2], N[2]);        // z high
+        add(lo[0], N[0], lo[1], a[1], a[2], b[2]);        // y low
+        add(lo[0], N[0], b[1], N[1], a[2], b[2]);         // y high
+        add(lo[0], a[0], a[1], b[1], a[2], b[2]);         // x low
+        add(b[0], N[0], a[1], b[1], a[2], b[2]);          // x high
+    }
+
+    void setup_passes() {
+        classify(false, leanH, shellH);
+        classify(true, leanE, shellE);
+        auto upload_boxes = [&](std::vector<Box> &v, Box *&dev, int &nblk) {
+            nblk = 0;
+            for (Box &b : v) { b.bstart = nblk; nblk += (int)blocks(b); }
+            if (!v.empty()) {
+                dev = alloc<Box>(v.size());
+                HIP_OK(hipMemcpyAsync(dev, v.data(), v.size() * sizeof(Box), hipMemcpyHostToDevice, stream));
+            }
+        };
+        upload_boxes(shellH, shellH_dev, shellH_blocks);
+        upload_boxes(shellE, shellE_dev, shellE_blocks);
+        // interior Drude-Lorentz / gain cells, in device order
+        std::vector<long long> sp;
+        auto interior = [&](int i, int j, int k) {
+            return i >= leanE.i0 && i < leanE.i1 && j >= leanE.j0 && j < leanE.j1 && k >= leanE.k0 && k < leanE.k1;
+        };
+        for (const std::vector<int> *cells : {&m.dl_cells, &m.gain_cells})
+            for (int f : *cells) {
+                const int i = f / ((m.Ny + 1) * (m.Nz + 1)), j = (f / (m.Nz + 1)) % (m.Ny + 1), k = f % (m.Nz + 1);
+                if (interior(i, j, k)) sp.push_back(dev_index(i, j, k));
+            }
+        std::sort(sp.begin(), sp.end());
+        nspecial = (int)sp.size();
+        if (nspecial) special = upload<long long>(sp);
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
+    void launch_update(bool h) {
+        if (!split_passes) {
+            if (h) launch_h(H_ALL, "update_h");
+            else launch_e(E_ALL, "update_e");
+            return;
+        }
+        const Box &L = h ? leanH : leanE;
+        const long long nl = blocks(L);
+        if (nl) {
+            timer.run(stream, h ? "lean_h" : "lean_e", [&] {
+                if (h) hipLaunchKernelGGL(k_lean_h<R>, dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L);
+                else hipLaunchKernelGGL(k_lean_e<R>, dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, m.has_gain ? 1 : 0);
+            });
+            HIP_OK(hipGetLastError());
+        }
+        const int nb = h ? shellH_blocks : shellE_blocks;
+        if (nb) {
+            timer.run(stream, h ? "shell_h" : "shell_e", [&] {
+                if (h) hipLaunchKernelGGL(k_shell_h<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellH_dev, (int)shellH.size());
+                else hipLaunchKernelGGL(k_shell_e<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellE_dev, (int)shellE.size());
+            });
+            HIP_OK(hipGetLastError());
+        }
+        if (!h && nspecial) {
+            timer.run(stream, "special_e", [&] {
+                hipLaunchKernelGGL(k_special_e<R>, dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, special, nspecial);
+            });
+            HIP_OK(hipGetLastError());
+        }
+    }
 
     void launch_h(unsigned parts, const char *name) {
-        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_h<R>, grid(), dim3(TX, TY), 0, stream, d, parts); });
+        const TileGrid g = tiles();
+        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_h<R>, dim3(g.gx * g.gy * g.gz), dim3(TX, TY), 0, stream, d, parts, g); });
         HIP_OK(hipGetLastError());
     }
     void launch_e(unsigned parts, const char *name) {
-        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_e<R>, grid(), dim3(TX, TY), 0, stream, d, parts); });
+        const TileGrid g = tiles();
+        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_e<R>, dim3(g.gx * g.gy * g.gz), dim3(TX, TY), 0, stream, d, parts, g); });
         HIP_OK(hipGetLastError());
     }
     void tfsf(bool h, int n) override {
@@ -312,15 +450,27 @@ public:
     }
     void ports(int n) override {
         if (n < 0 || n >= m.nt) throw Error("port time index out of range");
-        const int nf = (int)m.freqs.size();
-        const int do_dft = nf > 0 && (n % m.dft_stride == 0);
-        for (DevPort &p : dports) {
-            timer.run(stream, "ports", [&] {
-                hipLaunchKernelGGL(k_port<R>, dim3(p.nblk), dim3(256), 0, stream, d, p.cell, p.npts, n, p.samples,
-                                   p.dft, twE, twH, nf, m.nt, do_dft, p.ipart);
-            });
-            HIP_OK(hipGetLastError());
+        if (dports.empty()) return;
+        int slot = -1;
+        if (!m.freqs.empty() && n % m.dft_stride == 0) {
+            if (dft_pending == 0) dft_first = n;
+            slot = dft_pending++;
         }
+        timer.run(stream, "ports", [&] {
+            hipLaunchKernelGGL(k_port<R>, dim3(port_blocks), dim3(256), 0, stream, d, port_tab, (int)dports.size(), n, slot);
+        });
+        HIP_OK(hipGetLastError());
+        if (dft_pending == kDftBatch) dft_flush();
+    }
+    void dft_flush() {
+        if (!dft_pending) return;
+        const int nb = dft_pending;
+        dft_pending = 0;
+        timer.run(stream, "port_dft", [&] {
+            hipLaunchKernelGGL(k_dft<R>, dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
+                               dft_first, m.dft_stride, twE, twH, (int)m.freqs.size(), m.nt);
+        });
+        HIP_OK(hipGetLastError());
     }
     void basic_h() override { launch_h(H_BASIC, "basic_h"); }
     void cpml_h() override { if (m.has_pml) launch_h(H_PML, "cpml_h"); }
@@ -335,9 +485,9 @@ public:
         for (int c = 0; c < count; c++) {
             const int n = done;
             if (n >= m.nt) throw Error("run finished: all " + std::to_string(m.nt) + " steps done");
-            launch_h(H_ALL, "update_h");
+            launch_update(true);
             tfsf(true, n);
-            launch_e(E_ALL, "update_e");
+            launch_update(false);
             tfsf(false, n);
             sources(n);
             if (!dports.empty()) ports(n);
@@ -364,7 +514,7 @@ public:
         HIP_OK(hipStreamSynchronize(stream));
     }
     void port_samples(int q, double *host) override {
-        const DevPort &p = dports.at(q);
+        const PortDesc<R> &p = dports.at(q);
         if (!p.samples) throw Error("port samples were not recorded (record_ports = 0)");
         const size_t n = (size_t)done * p.npts * 6;
         std::vector<R> tmp(n);
@@ -373,8 +523,9 @@ public:
         for (size_t t = 0; t < n; t++) host[t] = (double)tmp[t];
     }
     void port_dft(int q, double *host) override {
-        const DevPort &p = dports.at(q);
+        const PortDesc<R> &p = dports.at(q);
         if (!p.dft) throw Error("no DFT frequencies configured");
+        dft_flush();
         const size_t nf = m.freqs.size(), np_ = p.npts;
         std::vector<double> tmp(np_ * nf * 12);
         HIP_OK(hipMemcpyAsync(tmp.data(), p.dft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -384,7 +535,7 @@ public:
                 for (size_t t = 0; t < np_; t++) host[(t * nf + f) * 12 + w] = tmp[(f * 12 + w) * np_ + t];
     }
     void port_intensity(int q, double *host) override {
-        const DevPort &p = dports.at(q);
+        const PortDesc<R> &p = dports.at(q);
         std::vector<double> tmp((size_t)done * p.nblk);
         if (!tmp.empty()) HIP_OK(hipMemcpyAsync(tmp.data(), p.ipart, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
@@ -400,8 +551,8 @@ public:
         std::vector<double> tmp(4 * std::max<size_t>(ng, 1));
         HIP_OK(hipMemcpyAsync(tmp.data(), d.gN, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
-        for (size_t s = 0; s < ng; s++)
-            for (int l = 0; l < 4; l++) host[s * 4 + l] = tmp[l * ng + s];
+        for (size_t r = 0; r < ng; r++)
+            for (int l = 0; l < 4; l++) host[r * 4 + l] = tmp[l * ng + gain_slot[r]];
     }
 };
 

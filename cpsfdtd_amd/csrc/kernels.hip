@@ -23,117 +23,90 @@ __device__ __forceinline__ long long cell_index(int px, int py, int i, int j, in
     return (long long)i + (long long)px * ((long long)j + (long long)py * k);
 }
 
-// H half-step: BasicFDTD.cpp:67-81 over [0,N)^3, then CPML.cpp:609-689 in the
-// slab tiles (x, then y, then z contributions, as the reference orders them).
+// ---------------------------------------------------------------------------
+// per-cell physics, shared by the grid kernels (module passes) and the
+// z-marching kernels (fused step).  Neighbour values arrive as arguments.
+// ---------------------------------------------------------------------------
+
+// H at one cell: BasicFDTD.cpp:67-81, then CPML.cpp:609-689 (x, then y, then
+// z slab contributions, in the reference's order).  *_i1/_j1/_k1 are E at the
+// +x/+y/+z neighbour.
 template <class R>
-__global__ __launch_bounds__(TX *TY) void k_update_h(Dev<R> d, unsigned parts)
+__device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, bool pml, int i, int j, int k, long long p,
+                                       R ex, R ey, R ez, R ex_j1, R ez_j1, R ex_k1, R ey_k1, R ey_i1, R ez_i1,
+                                       R &hx, R &hy, R &hz)
 {
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, k = blockIdx.z;
-    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-    bool tile_pml = false;
-    if ((parts & H_PML) && d.has_pml) {
-        tile_pml = (d.npml[0][0] && x0 < d.npml[0][0]) || (d.npml[0][1] && x0 + TX > d.Nx - d.npml[0][1]) ||
-                   (d.npml[1][0] && y0 < d.npml[1][0]) || (d.npml[1][1] && y0 + TY > d.Ny - d.npml[1][1]) ||
-                   (k < d.npml[2][0]) || (d.npml[2][1] && k >= d.Nz - d.npml[2][1]);
-    }
-    if (!(parts & H_BASIC) && !tile_pml) return;
-    if (i >= d.Nx || j >= d.Ny) return;
-    const long long p = cell_index(d.px, d.py, i, j, k);
-    const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
-    const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
-    const long long pk1 = p + d.slab;
-    const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
-    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    if (parts & H_BASIC) {
+    if (basic) {
         const int id = d.id[p];
         const R aH = d.aH[id], bH = d.bH[id];
-        hx = aH * hx - bH * (d.F[2][pj1] - ez) * d.st[S_cHxz][j] + bH * (d.F[1][pk1] - ey) * d.st[S_cHxy][k];
-        hy = aH * hy - bH * (d.F[0][pk1] - ex) * d.st[S_cHyx][k] + bH * (d.F[2][pi1] - ez) * d.st[S_cHyz][i];
-        hz = aH * hz + bH * (d.F[0][pj1] - ex) * d.st[S_cHzx][j] - bH * (d.F[1][pi1] - ey) * d.st[S_cHzy][i];
+        hx = aH * hx - bH * (ez_j1 - ez) * d.st[S_cHxz][j] + bH * (ey_k1 - ey) * d.st[S_cHxy][k];
+        hy = aH * hy - bH * (ex_k1 - ex) * d.st[S_cHyx][k] + bH * (ez_i1 - ez) * d.st[S_cHyz][i];
+        hz = aH * hz + bH * (ex_j1 - ex) * d.st[S_cHzx][j] - bH * (ey_i1 - ey) * d.st[S_cHzy][i];
     }
-    if (tile_pml) {
-        const R cH = d.cpmlH;
-        const bool fixed = d.hsign_fixed;
-        for (int e = 0; e < 2; e++) {  // x slabs: Hy += psi(dEz/dx), Hz -= psi(dEy/dx)
-            const int n = d.npml[0][e];
-            const int l = e == 0 ? i : i - (d.Nx - n);
-            if (n && l >= 0 && l < n) {
-                const R b = d.pb[1][0][e][l], c = d.pc[1][0][e][l];
-                const long long qi = l + (long long)n * (j + (long long)d.py * k);
-                R *q1 = d.q[1][0][e][0], *q2 = d.q[1][0][e][1];
-                const R a1 = b * q1[qi] + c * (d.F[2][pi1] - ez);
-                q1[qi] = a1;
-                hy = hy + a1 * cH;
-                const R a2 = b * q2[qi] + c * (d.F[1][pi1] - ey);
-                q2[qi] = a2;
-                hz = hz - a2 * cH;
-            }
-        }
-        for (int e = 0; e < 2; e++) {  // y slabs: Hz -/+ psi(dEx/dy), Hx +/- psi(dEz/dy)
-            const int n = d.npml[1][e];
-            const int l = e == 0 ? j : j - (d.Ny - n);
-            if (n && l >= 0 && l < n) {
-                const R b = d.pb[1][1][e][l], c = d.pc[1][1][e][l];
-                const long long qi = i + (long long)d.px * (l + (long long)n * k);
-                R *q1 = d.q[1][1][e][0], *q2 = d.q[1][1][e][1];
-                const R a1 = b * q1[qi] + c * (d.F[0][pj1] - ex);
-                q1[qi] = a1;
-                hz = fixed ? hz + a1 * cH : hz - a1 * cH;
-                const R a2 = b * q2[qi] + c * (d.F[2][pj1] - ez);
-                q2[qi] = a2;
-                hx = fixed ? hx - a2 * cH : hx + a2 * cH;
-            }
-        }
-        for (int e = 0; e < 2; e++) {  // z slabs: Hx -/+ psi(dEy/dz), Hy +/- psi(dEx/dz)
-            const int n = d.npml[2][e];
-            const int l = e == 0 ? k : k - (d.Nz - n);
-            if (n && l >= 0 && l < n) {
-                const R b = d.pb[1][2][e][l], c = d.pc[1][2][e][l];
-                const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
-                R *q1 = d.q[1][2][e][0], *q2 = d.q[1][2][e][1];
-                const R a1 = b * q1[qi] + c * (d.F[1][pk1] - ey);
-                q1[qi] = a1;
-                hx = fixed ? hx + a1 * cH : hx - a1 * cH;
-                const R a2 = b * q2[qi] + c * (d.F[0][pk1] - ex);
-                q2[qi] = a2;
-                hy = fixed ? hy - a2 * cH : hy + a2 * cH;
-            }
+    if (!pml) return;
+    const R cH = d.cpmlH;
+    const bool fixed = d.hsign_fixed;
+    for (int e = 0; e < 2; e++) {  // x slabs: Hy += psi(dEz/dx), Hz -= psi(dEy/dx)
+        const int n = d.npml[0][e];
+        const int l = e == 0 ? i : i - (d.Nx - n);
+        if (n && l >= 0 && l < n) {
+            const R b = d.pb[1][0][e][l], c = d.pc[1][0][e][l];
+            const long long qi = l + (long long)n * (j + (long long)d.py * k);
+            R *q1 = d.q[1][0][e][0], *q2 = d.q[1][0][e][1];
+            const R a1 = b * q1[qi] + c * (ez_i1 - ez);
+            q1[qi] = a1;
+            hy = hy + a1 * cH;
+            const R a2 = b * q2[qi] + c * (ey_i1 - ey);
+            q2[qi] = a2;
+            hz = hz - a2 * cH;
         }
     }
-    d.F[3][p] = hx;
-    d.F[4][p] = hy;
-    d.F[5][p] = hz;
+    for (int e = 0; e < 2; e++) {  // y slabs: Hz -/+ psi(dEx/dy), Hx +/- psi(dEz/dy)
+        const int n = d.npml[1][e];
+        const int l = e == 0 ? j : j - (d.Ny - n);
+        if (n && l >= 0 && l < n) {
+            const R b = d.pb[1][1][e][l], c = d.pc[1][1][e][l];
+            const long long qi = i + (long long)d.px * (l + (long long)n * k);
+            R *q1 = d.q[1][1][e][0], *q2 = d.q[1][1][e][1];
+            const R a1 = b * q1[qi] + c * (ex_j1 - ex);
+            q1[qi] = a1;
+            hz = fixed ? hz + a1 * cH : hz - a1 * cH;
+            const R a2 = b * q2[qi] + c * (ez_j1 - ez);
+            q2[qi] = a2;
+            hx = fixed ? hx - a2 * cH : hx + a2 * cH;
+        }
+    }
+    for (int e = 0; e < 2; e++) {  // z slabs: Hx -/+ psi(dEy/dz), Hy +/- psi(dEx/dz)
+        const int n = d.npml[2][e];
+        const int l = e == 0 ? k : k - (d.Nz - n);
+        if (n && l >= 0 && l < n) {
+            const R b = d.pb[1][2][e][l], c = d.pc[1][2][e][l];
+            const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
+            R *q1 = d.q[1][2][e][0], *q2 = d.q[1][2][e][1];
+            const R a1 = b * q1[qi] + c * (ey_k1 - ey);
+            q1[qi] = a1;
+            hx = fixed ? hx + a1 * cH : hx - a1 * cH;
+            const R a2 = b * q2[qi] + c * (ex_k1 - ex);
+            q2[qi] = a2;
+            hy = fixed ? hy - a2 * cH : hy + a2 * cH;
+        }
+    }
 }
 
-// E half-step over [1,N)^3 (x/y from 0 when periodic): four-level gain
-// polarisation from E^n, BasicFDTD.cpp:25-45 (ID 4 zeroed, ID 2 left to the
-// Drude-Lorentz ADE), the ADE for ID 2, the gain current and populations for
-// ID 5, then CPML.cpp:529-607 in the slab tiles.
+// E at one cell: four-level gain polarisation from E^n, BasicFDTD.cpp:25-45
+// (ID 4 zeroed, ID 2 left to the ADE), the Drude-Lorentz ADE for ID 2
+// (Drude_Lorentz.cpp:172-206, all three components, C1/C5 per F9/F10), the
+// gain current and FP64 populations for ID 5, then CPML.cpp:529-607.
+// *_im/_jm/_km are H (already updated this step) at the -x/-y/-z neighbour.
 template <class R>
-__global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts)
+__device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, bool pml, int i, int j, int k, long long p,
+                                       int id, R hx, R hy, R hz, R hx_jm, R hz_jm, R hx_km, R hy_km, R hy_im,
+                                       R hz_im, R (&E)[3])
 {
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, k = blockIdx.z;
-    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-    bool tile_pml = false;
-    if ((parts & E_PML) && d.has_pml) {
-        tile_pml = (d.npml[0][0] && x0 <= d.npml[0][0]) || (d.npml[0][1] && x0 + TX > d.Nx - d.npml[0][1]) ||
-                   (d.npml[1][0] && y0 <= d.npml[1][0]) || (d.npml[1][1] && y0 + TY > d.Ny - d.npml[1][1]) ||
-                   (k <= d.npml[2][0]) || (d.npml[2][1] && k >= d.Nz - d.npml[2][1]);
-    }
-    if (!(parts & ~E_PML) && !tile_pml) return;
-    if (i < d.ie0 || i >= d.Nx || j < d.je0 || j >= d.Ny || k < 1 || k >= d.Nz) return;
-    const long long p = cell_index(d.px, d.py, i, j, k);
-    const int id = d.id[p];
     const unsigned gparts = parts & (E_GAINPOL | E_GAINE | E_GAINPOP);
     const bool gain = gparts && d.ngain && id == 5;
     const bool dl = (parts & E_DL) && d.ndl && id == 2;
-    if (!(parts & (E_BASIC | E_PML)) && !gain && !dl) return;
-    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
-    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
-    const long long pkm = p - d.slab;
-    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
-    const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    R En[3] = {E[0], E[1], E[2]};  // E^n, kept for the gain populations
+    const R En[3] = {E[0], E[1], E[2]};  // E^n, kept for the gain populations
     int gs = -1;
     if (gain) {
         gs = d.aux[p];
@@ -154,25 +127,24 @@ __global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts)
             E[0] = 0; E[1] = 0; E[2] = 0;
         } else if (id != 2) {
             const R aE = d.aE[id], bE = d.bE[id];
-            E[0] = aE * E[0] + bE * (hz - d.F[5][pjm]) * d.st[S_cExz][j] - bE * (hy - d.F[4][pkm]) * d.st[S_cExy][k];
-            E[1] = aE * E[1] + bE * (hx - d.F[3][pkm]) * d.st[S_cEyx][k] - bE * (hz - d.F[5][pim]) * d.st[S_cEyz][i];
-            E[2] = aE * E[2] + bE * (hy - d.F[4][pim]) * d.st[S_cEzy][i] - bE * (hx - d.F[3][pjm]) * d.st[S_cEzx][j];
+            E[0] = aE * E[0] + bE * (hz - hz_jm) * d.st[S_cExz][j] - bE * (hy - hy_km) * d.st[S_cExy][k];
+            E[1] = aE * E[1] + bE * (hx - hx_km) * d.st[S_cEyx][k] - bE * (hz - hz_im) * d.st[S_cEyz][i];
+            E[2] = aE * E[2] + bE * (hy - hy_im) * d.st[S_cEzy][i] - bE * (hx - hx_jm) * d.st[S_cEzx][j];
         }
     }
-    if (dl) {  // Drude_Lorentz.cpp:172-206, all three components, C1/C5 per F9/F10
+    if (dl) {
         const int s = d.aux[p];
-        const R dA[3] = {hz - d.F[5][pjm], hx - d.F[3][pkm], hy - d.F[4][pim]};
+        const R dA[3] = {hz - hz_jm, hx - hx_km, hy - hy_im};
         const R cA[3] = {d.st[S_cExz][j], d.st[S_cEyx][k], d.st[S_cEzy][i]};
-        const R dB[3] = {hy - d.F[4][pkm], hz - d.F[5][pim], hx - d.F[3][pjm]};
+        const R dB[3] = {hy - hy_km, hz - hz_im, hx - hx_jm};
         const R cB[3] = {d.st[S_cExy][k], d.st[S_cEyz][i], d.st[S_cEzx][j]};
         const int nD = d.nD, nL = d.nL;
-        const long long ndl = d.ndl;
+        const long long ndl = d.ndl, vs = 3 * ndl;
         for (int c = 0; c < 3; c++) {
-            R *JD = d.dlst + (0LL * 3 + c) * ndl + s;               // var v at JD + v*3*ndl
-            R *JL = d.dlst + ((long long)nD * 3 + c) * ndl + s;
-            R *JL1 = d.dlst + ((long long)(nD + nL) * 3 + c) * ndl + s;
-            R *E1 = d.dlst + ((long long)(nD + 2 * nL) * 3 + c) * ndl + s;
-            const long long vs = 3 * ndl;
+            R *JD = d.dlst + (long long)c * ndl + s;  // variable v at JD + v*3*ndl
+            R *JL = JD + (long long)nD * vs;
+            R *JL1 = JL + (long long)nL * vs;
+            R *E1 = JL1 + (long long)nL * vs;
             const R temp = E[c];
             const R e1 = *E1;
             R t1 = 0, t2 = 0;
@@ -213,60 +185,280 @@ __global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts)
             d.gN[gs] = d.g_Ntot - n1 - n2 - n3;
         }
     }
-    if (tile_pml) {
-        const R cE = d.cpmlE;
-        for (int e = 0; e < 2; e++) {  // x slabs: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
-            const int n = d.npml[0][e];
-            const int l = e == 0 ? i - 1 : i - (d.Nx - n);
-            if (n && l >= 0 && l < n) {
-                const int ci = e == 0 ? i : l;
-                const R b = d.pb[0][0][e][ci], c = d.pc[0][0][e][ci];
-                const long long qi = l + (long long)n * (j + (long long)d.py * k);
-                R *q1 = d.q[0][0][e][0], *q2 = d.q[0][0][e][1];
-                const R a1 = b * q1[qi] + c * (hz - d.F[5][pim]);
-                q1[qi] = a1;
-                E[1] = E[1] - a1 * cE;
-                const R a2 = b * q2[qi] + c * (hy - d.F[4][pim]);
-                q2[qi] = a2;
-                E[2] = E[2] + a2 * cE;
-            }
-        }
-        for (int e = 0; e < 2; e++) {  // y slabs: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
-            const int n = d.npml[1][e];
-            const int l = e == 0 ? j - 1 : j - (d.Ny - n);
-            if (n && l >= 0 && l < n) {
-                const int cj = e == 0 ? j : l;
-                const R b = d.pb[0][1][e][cj], c = d.pc[0][1][e][cj];
-                const long long qi = i + (long long)d.px * (l + (long long)n * k);
-                R *q1 = d.q[0][1][e][0], *q2 = d.q[0][1][e][1];
-                const R a1 = b * q1[qi] + c * (hx - d.F[3][pjm]);
-                q1[qi] = a1;
-                E[2] = E[2] - a1 * cE;
-                const R a2 = b * q2[qi] + c * (hz - d.F[5][pjm]);
-                q2[qi] = a2;
-                E[0] = E[0] + a2 * cE;
-            }
-        }
-        for (int e = 0; e < 2; e++) {  // z slabs: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
-            const int n = d.npml[2][e];
-            const int l = e == 0 ? k - 1 : k - (d.Nz - n);
-            if (n && l >= 0 && l < n) {
-                const int ck = e == 0 ? k : l;
-                const R b = d.pb[0][2][e][ck], c = d.pc[0][2][e][ck];
-                const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
-                R *q1 = d.q[0][2][e][0], *q2 = d.q[0][2][e][1];
-                const R a1 = b * q1[qi] + c * (hy - d.F[4][pkm]);
-                q1[qi] = a1;
-                E[0] = E[0] - a1 * cE;
-                const R a2 = b * q2[qi] + c * (hx - d.F[3][pkm]);
-                q2[qi] = a2;
-                E[1] = E[1] + a2 * cE;
-            }
+    if (!pml) return;
+    const R cE = d.cpmlE;
+    for (int e = 0; e < 2; e++) {  // x slabs: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
+        const int n = d.npml[0][e];
+        const int l = e == 0 ? i - 1 : i - (d.Nx - n);
+        if (n && l >= 0 && l < n) {
+            const int ci = e == 0 ? i : l;
+            const R b = d.pb[0][0][e][ci], c = d.pc[0][0][e][ci];
+            const long long qi = l + (long long)n * (j + (long long)d.py * k);
+            R *q1 = d.q[0][0][e][0], *q2 = d.q[0][0][e][1];
+            const R a1 = b * q1[qi] + c * (hz - hz_im);
+            q1[qi] = a1;
+            E[1] = E[1] - a1 * cE;
+            const R a2 = b * q2[qi] + c * (hy - hy_im);
+            q2[qi] = a2;
+            E[2] = E[2] + a2 * cE;
         }
     }
+    for (int e = 0; e < 2; e++) {  // y slabs: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
+        const int n = d.npml[1][e];
+        const int l = e == 0 ? j - 1 : j - (d.Ny - n);
+        if (n && l >= 0 && l < n) {
+            const int cj = e == 0 ? j : l;
+            const R b = d.pb[0][1][e][cj], c = d.pc[0][1][e][cj];
+            const long long qi = i + (long long)d.px * (l + (long long)n * k);
+            R *q1 = d.q[0][1][e][0], *q2 = d.q[0][1][e][1];
+            const R a1 = b * q1[qi] + c * (hx - hx_jm);
+            q1[qi] = a1;
+            E[2] = E[2] - a1 * cE;
+            const R a2 = b * q2[qi] + c * (hz - hz_jm);
+            q2[qi] = a2;
+            E[0] = E[0] + a2 * cE;
+        }
+    }
+    for (int e = 0; e < 2; e++) {  // z slabs: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
+        const int n = d.npml[2][e];
+        const int l = e == 0 ? k - 1 : k - (d.Nz - n);
+        if (n && l >= 0 && l < n) {
+            const int ck = e == 0 ? k : l;
+            const R b = d.pb[0][2][e][ck], c = d.pc[0][2][e][ck];
+            const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
+            R *q1 = d.q[0][2][e][0], *q2 = d.q[0][2][e][1];
+            const R a1 = b * q1[qi] + c * (hy - hy_km);
+            q1[qi] = a1;
+            E[0] = E[0] - a1 * cE;
+            const R a2 = b * q2[qi] + c * (hx - hx_km);
+            q2[qi] = a2;
+            E[1] = E[1] + a2 * cE;
+        }
+    }
+}
+
+// tile-level CPML membership (workgroup-uniform)
+template <class R>
+__device__ __forceinline__ bool tile_in_pml_xy(const Dev<R> &d, int x0, int y0, bool efield)
+{
+    const int o = efield ? 1 : 0;  // E slabs start at index 1 on the low side
+    return (d.npml[0][0] && x0 < d.npml[0][0] + o) || (d.npml[0][1] && x0 + TX > d.Nx - d.npml[0][1]) ||
+           (d.npml[1][0] && y0 < d.npml[1][0] + o) || (d.npml[1][1] && y0 + TY > d.Ny - d.npml[1][1]);
+}
+template <class R>
+__device__ __forceinline__ bool plane_in_pml_z(const Dev<R> &d, int k, bool efield)
+{
+    const int o = efield ? 1 : 0;
+    return (d.npml[2][0] && k < d.npml[2][0] + o) || (d.npml[2][1] && k >= d.Nz - d.npml[2][1]);
+}
+
+// ---------------------------------------------------------------------------
+// grid kernels: one thread per cell, workgroup = 64 x 4 tile of one plane.
+// Used for the single-module passes (reference call granularity).
+// ---------------------------------------------------------------------------
+// grid kernels: one thread per cell, workgroups in linear tile order (x
+// fastest, then y, then z): all XCDs sweep the same planes together.
+struct TileGrid {
+    int gx, gy, gz;
+};
+__device__ __forceinline__ bool tile_of(const TileGrid &g, int &bx, int &by, int &bz)
+{
+    const int b = blockIdx.x;
+    if (b >= g.gx * g.gy * g.gz) return false;
+    bx = b % g.gx;
+    by = (b / g.gx) % g.gy;
+    bz = b / (g.gx * g.gy);
+    return true;
+}
+
+template <class R>
+__global__ __launch_bounds__(TX *TY) void k_update_h(Dev<R> d, unsigned parts, TileGrid g)
+{
+    int bx, by, k;
+    if (!tile_of(g, bx, by, k)) return;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+    const bool pml = (parts & H_PML) && d.has_pml && (tile_in_pml_xy(d, x0, y0, false) || plane_in_pml_z(d, k, false));
+    if (!(parts & H_BASIC) && !pml) return;
+    if (i >= d.Nx || j >= d.Ny) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
+    const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
+    const long long pk1 = p + d.slab;
+    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    h_cell(d, (parts & H_BASIC) != 0, pml, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
+           d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
+    d.F[3][p] = hx;
+    d.F[4][p] = hy;
+    d.F[5][p] = hz;
+}
+
+template <class R>
+__global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts, TileGrid g)
+{
+    int bx, by, k;
+    if (!tile_of(g, bx, by, k)) return;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+    const bool pml = (parts & E_PML) && d.has_pml && (tile_in_pml_xy(d, x0, y0, true) || plane_in_pml_z(d, k, true));
+    if (!(parts & ~E_PML) && !pml) return;
+    if (i < d.ie0 || i >= d.Nx || j < d.je0 || j >= d.Ny || k < 1 || k >= d.Nz) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const int id = d.id[p];
+    const bool gain = (parts & (E_GAINPOL | E_GAINE | E_GAINPOP)) && d.ngain && id == 5;
+    const bool dl = (parts & E_DL) && d.ndl && id == 2;
+    if (!(parts & (E_BASIC | E_PML)) && !gain && !dl) return;
+    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
+    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
+    const long long pkm = p - d.slab;
+    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    e_cell(d, parts, pml, i, j, k, p, id, d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm], d.F[3][pkm],
+           d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
     d.F[0][p] = E[0];
     d.F[1][p] = E[1];
     d.F[2][p] = E[2];
+}
+
+// ---------------------------------------------------------------------------
+// Fused-step kernels.  A half-step is split by cell class so that the hot
+// path stays lean (few registers, high occupancy):
+//   lean     the interior box (no CPML slab): plain Yee stencil; on the E side
+//            Drude-Lorentz / gain cells are skipped (left to the special pass)
+//   shell    the CPML slab boxes: full physics (stencil + psi + ADE + gain)
+//   special  interior Drude-Lorentz and gain cells, from a compact list
+// Every cell belongs to exactly one class, and within a half-step a cell's
+// update reads only the other field, so the passes are independent and each
+// cell's own sequence of operations is the reference's.
+// ---------------------------------------------------------------------------
+struct Box {
+    int i0, i1, j0, j1, k0, k1;  // cell ranges [lo, hi)
+    int tw, th;                  // tile width (x) and height (y), tw * th = 256
+    int xbase, gx, gy, bstart;   // x tiles start at xbase (aligned); block offset
+};
+
+// locate this workgroup's box, tile and cell; returns false for lanes outside
+__device__ __forceinline__ bool box_cell(const Box *boxes, int nbox, int &i, int &j, int &k, int &bi)
+{
+    int q = 0;
+    while (q + 1 < nbox && (int)blockIdx.x >= boxes[q + 1].bstart) q++;
+    const Box B = boxes[q];
+    const int t = blockIdx.x - B.bstart;
+    const int bx = t % B.gx, by = (t / B.gx) % B.gy;
+    k = B.k0 + t / (B.gx * B.gy);
+    const int lin = threadIdx.y * TX + threadIdx.x;
+    i = B.xbase + bx * B.tw + lin % B.tw;
+    j = B.j0 + by * B.th + lin / B.tw;
+    bi = q;
+    return i >= B.i0 && i < B.i1 && j < B.j1;
+}
+
+// interior H: BasicFDTD.cpp:67-81 only
+template <class R>
+__global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B)
+{
+    const int t = blockIdx.x;
+    const int bx = t % B.gx, by = (t / B.gx) % B.gy, k = B.k0 + t / (B.gx * B.gy);
+    const int i = B.xbase + bx * TX + threadIdx.x, j = B.j0 + by * TY + threadIdx.y;
+    if (i < B.i0 || i >= B.i1 || j >= B.j1) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
+    const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
+    const long long pk1 = p + d.slab;
+    const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
+    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    const int id = d.id[p];
+    const R aH = d.aH[id], bH = d.bH[id];
+    hx = aH * hx - bH * (d.F[2][pj1] - ez) * d.st[S_cHxz][j] + bH * (d.F[1][pk1] - ey) * d.st[S_cHxy][k];
+    hy = aH * hy - bH * (d.F[0][pk1] - ex) * d.st[S_cHyx][k] + bH * (d.F[2][pi1] - ez) * d.st[S_cHyz][i];
+    hz = aH * hz + bH * (d.F[0][pj1] - ex) * d.st[S_cHzx][j] - bH * (d.F[1][pi1] - ey) * d.st[S_cHzy][i];
+    d.F[3][p] = hx;
+    d.F[4][p] = hy;
+    d.F[5][p] = hz;
+}
+
+// interior E: BasicFDTD.cpp:25-45 for plain cells (ID 4 zeroed; ID 2 and, with a
+// gain medium, ID 5 are the special pass's)
+template <class R>
+__global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, int skip5)
+{
+    const int t = blockIdx.x;
+    const int bx = t % B.gx, by = (t / B.gx) % B.gy, k = B.k0 + t / (B.gx * B.gy);
+    const int i = B.xbase + bx * TX + threadIdx.x, j = B.j0 + by * TY + threadIdx.y;
+    if (i < B.i0 || i >= B.i1 || j >= B.j1) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const int id = d.id[p];
+    if (id == 2 || (skip5 && id == 5)) return;
+    if (id == 4) {
+        d.F[0][p] = 0;
+        d.F[1][p] = 0;
+        d.F[2][p] = 0;
+        return;
+    }
+    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
+    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
+    const long long pkm = p - d.slab;
+    const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    const R aE = d.aE[id], bE = d.bE[id];
+    const R e0 = aE * d.F[0][p] + bE * (hz - d.F[5][pjm]) * d.st[S_cExz][j] - bE * (hy - d.F[4][pkm]) * d.st[S_cExy][k];
+    const R e1 = aE * d.F[1][p] + bE * (hx - d.F[3][pkm]) * d.st[S_cEyx][k] - bE * (hz - d.F[5][pim]) * d.st[S_cEyz][i];
+    const R e2 = aE * d.F[2][p] + bE * (hy - d.F[4][pim]) * d.st[S_cEzy][i] - bE * (hx - d.F[3][pjm]) * d.st[S_cEzx][j];
+    d.F[0][p] = e0;
+    d.F[1][p] = e1;
+    d.F[2][p] = e2;
+}
+
+// CPML slab boxes: full physics
+template <class R>
+__global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, const Box *boxes, int nbox)
+{
+    int i, j, k, bi;
+    if (!box_cell(boxes, nbox, i, j, k, bi)) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
+    const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
+    const long long pk1 = p + d.slab;
+    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    h_cell(d, true, true, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1], d.F[0][pk1],
+           d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
+    d.F[3][p] = hx;
+    d.F[4][p] = hy;
+    d.F[5][p] = hz;
+}
+
+template <class R>
+__device__ __forceinline__ void e_full(const Dev<R> &d, int i, int j, int k, bool pml)
+{
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
+    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
+    const long long pkm = p - d.slab;
+    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    e_cell(d, E_ALL, pml, i, j, k, p, d.id[p], d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm],
+           d.F[3][pkm], d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
+    d.F[0][p] = E[0];
+    d.F[1][p] = E[1];
+    d.F[2][p] = E[2];
+}
+
+template <class R>
+__global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, const Box *boxes, int nbox)
+{
+    int i, j, k, bi;
+    if (!box_cell(boxes, nbox, i, j, k, bi)) return;
+    e_full(d, i, j, k, true);
+}
+
+// interior Drude-Lorentz and gain cells (device cell indices)
+template <class R>
+__global__ __launch_bounds__(256) void k_special_e(Dev<R> d, const long long *cells, int n)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const long long p = cells[t];
+    const int k = (int)(p / d.slab);
+    const long long r = p - (long long)k * d.slab;
+    const int j = (int)(r / d.px), i = (int)(r - (long long)j * d.px);
+    e_full(d, i, j, k, false);
 }
 
 // TF_SF.cpp:966-986
@@ -314,43 +506,79 @@ __global__ void k_sources(Dev<R> d, const long long *cell, const int *comp, cons
     F[cell[q]] = F[cell[q]] + (R)wave[(long long)q * stride + n];
 }
 
-// Port gather (Ports.cpp:88-99, 342-353), running FP64 DFT and the per-step
-// sum of |E|^2 (Ports.cpp:512-522), reduced across the workgroup with
-// wavefront shuffles into one partial per workgroup (fixed order).
+// One port of a simulation as the port kernels see it.  Points of all ports
+// are handled by one launch; port q owns workgroups [bstart, bstart + nblk).
 template <class R>
-__global__ __launch_bounds__(256) void k_port(Dev<R> d, const long long *cell, int npts, int n, R *samples,
-                                              double *dft, const double *twE, const double *twH, int nf, int ntot,
-                                              int do_dft, double *ipart)
+struct PortDesc {
+    const long long *cell;
+    R *samples;   // [step][point][6] when recording, else null
+    R *batch;     // [slot][6][point] samples awaiting the DFT, null without DFT
+    double *dft;  // [f][comp][re,im][point] FP64 accumulators
+    double *ipart;// [step][nblk] per-workgroup partial sums of |E|^2
+    int npts, nblk, bstart, pad;
+};
+
+// Port gather (Ports.cpp:88-99, 342-353) into the time-series record and the
+// DFT batch, and the per-step sum of |E|^2 (Ports.cpp:512-522) reduced across
+// the workgroup with wavefront shuffles into one partial per workgroup.
+template <class R>
+__global__ __launch_bounds__(256) void k_port(Dev<R> d, const PortDesc<R> *ports, int nports, int n, int slot)
 {
     __shared__ double red[4];
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    int q = 0;
+    while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
+    const PortDesc<R> P = ports[q];
+    const int blk = blockIdx.x - P.bstart;
+    const int t = blk * 256 + threadIdx.x;
     double isq = 0.0;
-    if (t < npts) {
-        const long long p = cell[t];
+    if (t < P.npts) {
+        const long long p = P.cell[t];
         R v[6];
         for (int c = 0; c < 6; c++) v[c] = d.F[c][p];
-        if (samples)
-            for (int c = 0; c < 6; c++) samples[((long long)n * npts + t) * 6 + c] = v[c];
+        if (P.samples)
+            for (int c = 0; c < 6; c++) P.samples[((long long)n * P.npts + t) * 6 + c] = v[c];
+        if (slot >= 0)
+            for (int c = 0; c < 6; c++) P.batch[((long long)slot * 6 + c) * P.npts + t] = v[c];
         isq = (double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (do_dft) {  // accumulators [f][comp][re,im][point]: consecutive lanes, consecutive words
-            const long long np_ = npts;
-            for (int f = 0; f < nf; f++) {
-                const double *wE = twE + ((long long)f * ntot + n) * 2, *wH = twH + ((long long)f * ntot + n) * 2;
-                double *acc = dft + (long long)f * 12 * np_ + t;
-                for (int c = 0; c < 6; c++) {
-                    const double *w = c < 3 ? wE : wH;
-                    double *re = acc + (2 * c) * np_, *im = acc + (2 * c + 1) * np_;
-                    *re = *re + (double)v[c] * w[0];
-                    *im = *im - (double)v[c] * w[1];
-                }
-            }
-        }
     }
     for (int off = 32; off > 0; off >>= 1) isq += __shfl_down(isq, off, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) red[wave] = isq;
     __syncthreads();
-    if (threadIdx.x == 0) ipart[(long long)n * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) P.ipart[(long long)n * P.nblk + blk] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// Running DFT over a batch of nb sampled steps n0, n0+stride, ...: each FP64
+// accumulator is read once, gets the batch's terms in step order (the same
+// additions, in the same order, as accumulating every sampled step), and is
+// written once.  E samples carry phase 2 pi f (n+1) dt, H samples (n+1/2) dt.
+template <class R>
+__global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nports, int nb, int n0, int stride,
+                                             const double *twE, const double *twH, int nf, int ntot)
+{
+    int q = 0;
+    while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
+    const PortDesc<R> P = ports[q];
+    const int t = (blockIdx.x - P.bstart) * 256 + threadIdx.x;
+    if (t >= P.npts) return;
+    const long long np_ = P.npts;
+    for (int f = 0; f < nf; f++) {
+        double *acc = P.dft + (long long)f * 12 * np_ + t;
+        double a[12];
+        for (int w = 0; w < 12; w++) a[w] = acc[w * np_];
+        for (int b = 0; b < nb; b++) {
+            const long long n = n0 + (long long)b * stride;
+            const double *wE = twE + ((long long)f * ntot + n) * 2, *wH = twH + ((long long)f * ntot + n) * 2;
+            const R *x = P.batch + (long long)b * 6 * np_ + t;
+            for (int c = 0; c < 6; c++) {
+                const double *w = c < 3 ? wE : wH;
+                const double v = (double)x[c * np_];
+                a[2 * c] = a[2 * c] + v * w[0];
+                a[2 * c + 1] = a[2 * c + 1] - v * w[1];
+            }
+        }
+        for (int w = 0; w < 12; w++) acc[w * np_] = a[w];
+    }
 }
 
 }  // namespace cfd

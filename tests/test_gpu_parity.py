@@ -73,6 +73,17 @@ def test_mixed_f64_bit_exact(tmp_path, module, flags):
         assert rel_l2(e.port_intensity(q), o.port_intensity(q)) < 1e-12
 
 
+@pytest.mark.parametrize("form", ["split", "grid"])
+def test_update_forms_bit_exact(tmp_path, monkeypatch, form):
+    """Both fused-step forms: lean/shell/special passes and the single
+    full-physics grid kernel per half-step."""
+    monkeypatch.setenv("CFD_UPDATE_KERNEL", form)
+    case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
+    e, o = run_both(case, 40)
+    assert_fields_equal(e, o, form)
+    assert np.array_equal(e.gain_pop(), o.gain_pop())
+
+
 def test_dl_gain_f64_bit_exact(tmp_path):
     case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
     e, o = run_both(case, 50)
