@@ -255,16 +255,34 @@ public:
         }
         // TF/SF
         if (m.has_tfsf) {
+            // one thread per corrected value; targets sorted by (component,
+            // device address) so a wavefront's read-modify-writes are contiguous
+            // (each target keeps its terms in the reference's face order)
             std::vector<TfTarget> te, th;
             for (const TfsfTarget &t : m.tfE) te.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
             for (const TfsfTarget &t : m.tfH) th.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
+            auto by_addr = [](const TfTarget &a, const TfTarget &b) {
+                return a.comp != b.comp ? a.comp < b.comp : a.cell < b.cell;
+            };
+            std::sort(te.begin(), te.end(), by_addr);
+            std::sort(th.begin(), th.end(), by_addr);
+            // terms laid out in sorted-target order too
+            std::vector<double> c, dl, pr;
+            for (std::vector<TfTarget> *tv : {&te, &th})
+                for (TfTarget &t : *tv) {
+                    const int first = (int)c.size();
+                    for (int q = t.first; q < t.first + t.count; q++) {
+                        c.push_back(m.tf_terms[q].coef);
+                        dl.push_back(m.tf_terms[q].delay);
+                        pr.push_back(m.tf_terms[q].prof);
+                    }
+                    t.first = first;
+                }
             ntfE = (int)te.size(); ntfH = (int)th.size();
             tfE = alloc<TfTarget>(te.size());
             tfH = alloc<TfTarget>(th.size());
             HIP_OK(hipMemcpyAsync(tfE, te.data(), te.size() * sizeof(TfTarget), hipMemcpyHostToDevice, stream));
             HIP_OK(hipMemcpyAsync(tfH, th.data(), th.size() * sizeof(TfTarget), hipMemcpyHostToDevice, stream));
-            std::vector<double> c, dl, pr;
-            for (const TfsfTerm &t : m.tf_terms) { c.push_back(t.coef); dl.push_back(t.delay); pr.push_back(t.prof); }
             tcoef = upload<double>(c); tdelay = upload<double>(dl); tprof = upload<double>(pr);
             einc = upload<double>(m.Einc); hinc = upload<double>(m.Hinc);
             tablen = (int)m.Einc.size();

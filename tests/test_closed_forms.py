@@ -1,0 +1,254 @@
+"""Closed forms and invariants that pin the oracle where the reference has no
+runnable code (its Drude-Lorentz and gain classes are 2-D only, PBC is a stub,
+ports have no DFT).  The GPU parity tests carry these pins to the engine bit
+for bit.  Each check is chosen so that a plausible mistake (a flipped sign, a
+dropped time step, a wrong index) fails it."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import cases
+from cpsfdtd_amd.case import C0, EPS0, MU0, CaseWriter, Shape, FIXED
+
+DL_POLES = dict(eps_inf=3.7, drude=[(1.37e16, 8.0e14)], lorentz=[(4.2e15, 1e15, 1.0), (6.1e15, 2e15, 0.5)])
+
+
+def dl_case(tmp_path, d=10e-9, **kw):
+    p = dict(DL_POLES)
+    p.update(kw)
+    w = CaseWriter(str(tmp_path))
+    w.materials([(1, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0)])
+    w.grid((8, 8, 8), (d, d, d), [])
+    w.drude_lorentz(p["eps_inf"], p["drude"], p["lorentz"])
+    return w.run(10), p
+
+
+def h_discrete(z, dt, p):
+    """E(z)/C(z) of the ADE as written down from its difference equations:
+    trapezoidal Ampere at n+1/2, trapezoidal Drude J' + g J = e0 wp^2 E,
+    central-difference Lorentz J'' + 2 d J' + w0^2 J = e0 De w0^2 E'."""
+    amp = EPS0 * p["eps_inf"] * (z - 1) / dt
+    chi = 0
+    for wp, g in p["drude"]:
+        chi = chi + EPS0 * wp * wp * (z + 1) / 2 / ((z - 1) / dt + g * (z + 1) / 2)
+    for w0, dl, de in p["lorentz"]:
+        lhs = (z - 2 + 1 / z) / dt ** 2 + dl * (z - 1 / z) / dt + w0 * w0
+        chi = chi + EPS0 * de * w0 * w0 * (z - 1 / z) / (2 * dt) / lhs
+    return 1.0 / (amp + (z + 1) / 2 * chi)
+
+
+def h_continuous(w, p):
+    """1 / (i w e0 eps(w)) with eps = eps_inf + Drude + Lorentz (e^{+iwt})."""
+    iw = 1j * w
+    s = iw * EPS0 * p["eps_inf"]
+    for wp, g in p["drude"]:
+        s = s + EPS0 * wp * wp / (g + iw)
+    for w0, dl, de in p["lorentz"]:
+        s = s + EPS0 * de * w0 * w0 * iw / (w0 * w0 - w * w + 2j * dl * w)
+    return 1.0 / s
+
+
+def test_drude_lorentz_single_cell_transfer_function(oracle_lib, tmp_path):
+    """The oracle's ADE (coefficients C1..C6, kappa, beta, alpha, xi, gamma of
+    Drude_Lorentz.cpp:74-108 with F9/F10 fixed) driven by a windowed curl
+    sequence reproduces the z-domain transfer function of its defining
+    difference equations."""
+    case, p = dl_case(tmp_path)
+    o = oracle_lib.OracleSim(case)
+    dt = o.dt
+    n = np.arange(60000)
+    w1 = 2 * math.pi * C0 / 600e-9
+    c = np.exp(-((n - 1200) / 300.0) ** 2) * np.sin(w1 * n * dt)
+    y = o.dl_0d(c)
+    assert abs(y[-100:]).max() < 1e-12 * abs(y).max()  # decayed inside the window
+    for lam in (450e-9, 550e-9, 600e-9, 700e-9):
+        w = 2 * math.pi * C0 / lam
+        z = np.exp(1j * w * dt)
+        Y = np.sum(y * z ** (-n))
+        Cz = np.sum(c * z ** (-n))
+        got = Y / (z * Cz)
+        want = h_discrete(z, dt, p)
+        assert abs(got - want) / abs(want) < 1e-8, (lam, got, want)
+
+
+def dl_recurrence(c, dt, C1, C2, C3, C4, C5, C6, kap, beta, alpha, xi, gam):
+    """Single-cell ADE recurrence with given coefficients (the form of
+    Drude_Lorentz.cpp:172-183), for checking that the transfer-function test
+    rejects a wrong coefficient."""
+    nD, nL = len(C4), len(C5)
+    JD, JL, JL1 = np.zeros(nD), np.zeros(nL), np.zeros(nL)
+    E = E1 = 0.0
+    out = np.empty(len(c))
+    for n in range(len(c)):
+        t = E
+        En = C1 * E + C2 * E1 + C3 * c[n] - np.dot(C4, JD) - np.dot(C5, JL) - np.dot(C6, JL1)
+        JD = kap * JD + beta * (En + t)
+        JL, JL1 = alpha * JL + xi * JL1 + gam * (En - E1) / 2.0 / dt, JL
+        E1, E = t, En
+        out[n] = E
+    return out
+
+
+def test_transfer_function_rejects_reference_c1_and_c5(oracle_lib, tmp_path):
+    """The reference's C1 (sign of the loss terms, F9) and C5 (no dt, F10) give
+    a response that leaves the transfer function by orders of magnitude more
+    than the tolerance used above, so that check does discriminate."""
+    case, p = dl_case(tmp_path)
+    o = oracle_lib.OracleSim(case)
+    dt = o.dt
+    nD, nL = len(p["drude"]), len(p["lorentz"])
+    co = o.dl_coefs()
+    C1, C2, C3 = co[0:3]
+    k = 3
+    C4 = co[k:k + nD]; k += nD
+    C5 = co[k:k + nL]; k += nL
+    C6 = co[k:k + nL]; k += nL
+    kap = co[k:k + nD]; k += nD
+    beta = co[k:k + nD]; k += nD
+    alpha = co[k:k + nL]; k += nL
+    xi = co[k:k + nL]; k += nL
+    gam = co[k:k + nL]
+    n = np.arange(12000)
+    w1 = 2 * math.pi * C0 / 600e-9
+    c = np.exp(-((n - 1200) / 300.0) ** 2) * np.sin(w1 * n * dt)
+    base = dl_recurrence(c, dt, C1, C2, C3, C4, C5, C6, kap, beta, alpha, xi, gam)
+    np.testing.assert_allclose(base, o.dl_0d(c), rtol=1e-12, atol=1e-12 * np.abs(base).max())
+    den = 2 * dt / C3
+    wrong_c1 = (2 * EPS0 * p["eps_inf"] + dt * beta.sum()) / den
+    wrong_c5 = (1.0 + alpha) / den
+    w = 2 * math.pi * C0 / 600e-9
+    z = np.exp(1j * w * dt)
+    want = h_discrete(z, dt, p)
+    for bad in (dl_recurrence(c, dt, wrong_c1, C2, C3, C4, C5, C6, kap, beta, alpha, xi, gam),
+                dl_recurrence(c, dt, C1, C2, C3, C4, wrong_c5, C6, kap, beta, alpha, xi, gam)):
+        Y = np.sum(bad * z ** (-n))
+        with np.errstate(invalid="ignore", over="ignore"):
+            got = Y / (z * np.sum(c * z ** (-n)))
+            rel = abs(got - want) / abs(want)
+        assert not rel <= 1e-3, rel  # an unstable (nan/inf) response counts as rejected
+
+
+def test_drude_lorentz_continuous_limit(oracle_lib, tmp_path):
+    """As dt -> 0 the discrete response converges to the continuous
+    Drude-Lorentz medium at second order: pins the physical meaning of the
+    file's parameters (rad/s, delta as the half linewidth, Delta-eps).  The
+    curl sample c[n] sits at (n+1/2) dt and E^{n} at n dt, so the continuous
+    ratio carries exp(i w dt / 2); without it the error is first order."""
+    case, p = dl_case(tmp_path)
+    dt = oracle_lib.OracleSim(case).dt
+    for lam in (500e-9, 800e-9, 1200e-9):
+        w = 2 * math.pi * C0 / lam
+        hc = h_continuous(w, p)
+        err = [abs(h_discrete(np.exp(1j * w * h), h, p) * np.exp(0.5j * w * h) - hc) / abs(hc)
+               for h in (dt, dt / 2, dt / 4, dt / 8)]
+        assert err[0] < 2e-3, (lam, err)
+        for a, b in zip(err[1:], err[2:]):
+            assert 3.5 < a / b < 4.5, (lam, err)
+
+
+def gain_case(tmp_path, pump=1e13, t32=1e-13, t21=1e-12, t10=1e-13, ntot=3.3e24, d=10e-9):
+    w = CaseWriter(str(tmp_path))
+    w.materials([(1, 1, 0, 0)] * 6)
+    w.grid((8, 8, 8), (d, d, d), [])
+    w.gain(600e-9, 1e14, t32, t21, t10, ntot, pump)
+    return w.run(10)
+
+
+def test_gain_relaxation_matches_rate_equations(oracle_lib, tmp_path):
+    """With no field the four-level populations follow the linear rate
+    equations dN3 = Wp N0 - N3/t32, dN2 = N3/t32 - N2/t21, dN1 = N2/t21 - N1/t10,
+    N0 = Ntot - N1 - N2 - N3 (matrix exponential).  The decays are
+    Crank-Nicolson and the pump term explicit (the oracle's 3-D restatement of
+    the FourLevelLaser.cpp:296-316 population step),
+    so the error at a fixed physical time halves with dt (dt follows the cell
+    size through the Courant limit)."""
+    pump, t32, t21, t10, ntot = 1e13, 1e-13, 1e-12, 1e-13, 3.3e24
+    A = np.array([[-pump, 1 / t10, 0, 0],
+                  [0, -1 / t10, 1 / t21, 0],
+                  [0, 0, -1 / t21, 1 / t32],
+                  [pump, 0, 0, -1 / t32]])
+    N0 = np.array([ntot, 0, 0, 0])
+    err = []
+    for k, d in enumerate((10e-9, 5e-9, 2.5e-9)):
+        sub = tmp_path / ("d%d" % k)
+        sub.mkdir()
+        o = oracle_lib.OracleSim(gain_case(sub, pump, t32, t21, t10, ntot, d=d))
+        nsteps = 6000 << k
+        pop, P = o.gain_0d(np.zeros(nsteps))
+        assert np.all(P == 0.0)
+        np.testing.assert_allclose(pop.sum(axis=1), ntot, rtol=1e-13)
+        want = scipy.linalg.expm(A * nsteps * o.dt) @ N0
+        assert want[3] > 0.01 * ntot  # the pump really moved population
+        err.append(np.abs(pop[-1] - want).max() / np.abs(want[1:]).max())
+    assert err[0] < 1e-3, err
+    for a, b in zip(err, err[1:]):
+        assert 1.8 < a / b < 2.2, err
+
+
+def test_gain_population_conserved_under_drive(oracle_lib, tmp_path):
+    o = oracle_lib.OracleSim(gain_case(tmp_path))
+    n = np.arange(8000)
+    w = 2 * math.pi * C0 / 600e-9
+    pop, P = o.gain_0d(1e8 * np.sin(w * n * o.dt))
+    np.testing.assert_allclose(pop.sum(axis=1), 3.3e24, rtol=1e-13)
+    assert np.abs(P).max() > 0
+
+
+def test_gain_drive_couples_to_populations(oracle_lib, tmp_path):
+    """A pumped medium driven at its emission line exchanges energy with the
+    field through the N1-N2 transition: the drive changes N2 relative to the
+    undriven run, and without a drive the polarisation stays exactly zero."""
+    o = oracle_lib.OracleSim(gain_case(tmp_path, pump=1e13))
+    n = np.arange(30000)
+    w = 2 * math.pi * C0 / 600e-9
+    drive = 5e8 * np.sin(w * n * o.dt)
+    popd, Pd = o.gain_0d(drive)
+    pop0, P0 = o.gain_0d(np.zeros_like(drive))
+    assert np.all(P0 == 0.0) and np.abs(Pd).max() > 0
+    assert abs(popd[-1, 2] - pop0[-1, 2]) > 1e-9 * pop0[-1, 2]
+
+
+def pbc_case(tmp_path, nx=12, ny=10, nz=48):
+    d = (10e-9, 10e-9, 10e-9)
+    w = CaseWriter(str(tmp_path))
+    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (4.0, 1, 0.05, 0)])
+    w.grid((nx, ny, nz), d, [Shape(1, "cuboid", x0=0.0, y0=0.0, z0=20 * d[2], a=(nx + 1) * d[0], b=(ny + 1) * d[1], c=6 * d[2]),
+                            Shape(2, "cuboid", x0=0.0, y0=0.0, z0=30 * d[2], a=(nx + 1) * d[0], b=(ny + 1) * d[1], c=3 * d[2])])
+    w.cpml(((0, 0), (0, 0), (8, 8)), m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.tfsf(400e-9, 200e-9, 1e3, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, ((0, 0), (0, 0), (12, 12)), 300)
+    return w.run(300, periodic=3, flags=FIXED)
+
+
+def test_periodic_plane_wave_translation_invariant(oracle_lib, tmp_path):
+    """Normal incidence on an x/y-uniform layered stack in a periodic cell:
+    every column carries the same field, exactly (a wrong wrap index breaks it)."""
+    o = oracle_lib.OracleSim(pbc_case(tmp_path))
+    o.step(300)
+    fields = [o.field(c)[:o.N[0], :o.N[1], :] for c in range(6)]
+    assert np.abs(fields[1]).max() > 1e-3 and np.abs(fields[3]).max() > 0  # Ey, Hx carry the wave
+    for c, f in enumerate(fields):
+        assert np.array_equal(f, np.broadcast_to(f[:1, :1, :], f.shape)), c
+
+
+def test_port_dft_equals_numpy(oracle_lib, tmp_path):
+    """The running DFT at the ports equals numpy's DFT of the recorded samples
+    at the same stride: X(f) = sum x_n exp(-2 pi i f t_n), t_n = (n+1) dt for E
+    and (n+1/2) dt for H."""
+    freqs = (4.0e14, 5.5e14, 7.5e14)
+    case = cases.mixed(str(tmp_path), freqs=freqs, stride=3, flags=FIXED, nt=90)
+    o = oracle_lib.OracleSim(case)
+    o.step(90)
+    for q in range(2):
+        x = o.port_samples(q)
+        got = o.port_dft(q)
+        n = np.arange(0, 90, 3)
+        for fi, f in enumerate(freqs):
+            tE = (n + 1.0) * o.dt
+            tH = (n + 0.5) * o.dt
+            for c in range(6):
+                t = tE if c < 3 else tH
+                want = np.sum(x[n, :, c] * np.exp(-2j * np.pi * f * t)[:, None], axis=0)
+                scale = np.abs(x[:, :, c]).max() * len(n) + 1e-300
+                assert np.abs(got[:, fi, c] - want).max() <= 1e-12 * scale
