@@ -61,19 +61,23 @@ def algorithmic_bytes(geo, S, nd_vars):
     """Algorithmic HBM bytes per launch of each update pass: an updated cell
     reads its own 6 field values and its ID byte and writes 3 field values
     (neighbour values are reuse, not traffic); each slab a CPML cell lies in
-    adds 2 psi values read and written; a Drude-Lorentz cell adds its 4-byte
-    slot and 3 x nd_vars auxiliaries read and written (plus an 8-byte cell
-    index in the special pass); a lean-pass cell it skips costs its ID byte."""
+    adds 2 psi values read and written.  Drude-Lorentz cells are counted in
+    the special pass (cell index 8 B, slot 4 B, 3 x nd_vars auxiliaries read
+    and written); the pass that leaves them alone is charged their ID byte."""
     per_cell = 9 * S + 1
-    dl_bytes = 4 + 3 * nd_vars * 2 * S
+    dl_bytes = 8 + 4 + 3 * nd_vars * 2 * S
     h, e = geo["h"], geo["e"]
-    nsp, nsh_dl = geo["dl_interior"], geo["dl_shell"]
+    nsp, nsh = geo["dl_interior"], geo["dl_shell"]
     return {
+        # merged form: one launch per half-step over every cell
+        "full_h": (h["lean"] + h["shell"]) * per_cell + h["slab"] * 4 * S,
+        "full_e": (e["lean"] + e["shell"] - nsp - nsh) * per_cell + nsp + nsh + e["slab"] * 4 * S,
+        # split form: interior box, CPML shell boxes
         "lean_h": h["lean"] * per_cell,
         "shell_h": h["shell"] * per_cell + h["slab"] * 4 * S,
         "lean_e": (e["lean"] - nsp) * per_cell + nsp,
-        "shell_e": e["shell"] * per_cell + e["slab"] * 4 * S + nsh_dl * dl_bytes,
-        "special_e": nsp * (per_cell + 8 + dl_bytes),
+        "shell_e": (e["shell"] - nsh) * per_cell + nsh + e["slab"] * 4 * S,
+        "special_e": (nsp + nsh) * (per_cell + dl_bytes),
     }
 
 

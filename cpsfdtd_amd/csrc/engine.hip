@@ -165,7 +165,10 @@ public:
     }
 
     void setup() {
-        if (const char *v = std::getenv("CFD_UPDATE_KERNEL")) split_passes = std::string(v) != "grid";
+        if (const char *v = std::getenv("CFD_UPDATE_KERNEL")) {
+            const std::string f(v);
+            form = f == "grid" ? kFormGrid : f == "merged" ? kFormMerged : kFormSplit;
+        }
         d.Nx = m.Nx; d.Ny = m.Ny; d.Nz = m.Nz;
         d.px = ((m.Nx + 1) + 31) / 32 * 32;
         d.py = m.Ny + 1;
@@ -329,14 +332,20 @@ public:
         g.gz = m.Nz;
         return g;
     }
-    // Fused-step update passes: lean interior box, CPML shell boxes, special
-    // (Drude-Lorentz / gain) interior cells.  CFD_UPDATE_KERNEL=grid selects the
-    // single full-physics grid kernel per half-step instead (A/B comparison).
-    bool split_passes = true;
+    // Fused-step update forms (CFD_UPDATE_KERNEL selects one, for A/B checks):
+    //   split (default)   lean interior box, CPML shell boxes, special cells
+    //   merged            one launch per half-step over all tiles, lean stencil
+    //                     on interior tiles, CPML terms on the others, plus the
+    //                     special pass for Drude-Lorentz / gain cells
+    //   grid              the single full-physics grid kernel per half-step
+    enum { kFormMerged, kFormSplit, kFormGrid };
+    int form = kFormSplit;
     Box leanH{}, leanE{};
     std::vector<Box> shellH, shellE;
-    Box *shellH_dev = nullptr, *shellE_dev = nullptr;
+    BoxSet shellH_set{}, shellE_set{};
     int shellH_blocks = 0, shellE_blocks = 0;
+    LeanStretch<R> leanH_st{}, leanE_st{};
+    bool leanH_cs = false, leanE_cs = false;
     long long *special = nullptr;
     int nspecial = 0;
 
@@ -364,33 +373,59 @@ public:
             b[ax] = N[ax] - n2;
         }
         lean = make_box(a[0], b[0], a[1], b[1], a[2], b[2], TX);
-        auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1) {
+        auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1, int pml) {
             const int tw = (i1 - i0 <= 16) ? 16 : TX;
             Box bx = make_box(i0, i1, j0, j1, k0, k1, tw);
+            bx.pml = pml;
             if (!empty(bx)) shell.push_back(bx);
         };
-        add(lo[0], N[0], lo[1], N[1], lo[2], a[2]);       // z low
-        add(lo[0], N[0], lo[1], N[1], b[2], N[2]);        // z high
-        add(lo[0], N[0], lo[1], a[1], a[2], b[2]);        // y low
-        add(lo[0], N[0], b[1], N[1], a[2], b[2]);         // y high
-        add(lo[0], a[0], a[1], b[1], a[2], b[2]);         // x low
-        add(b[0], N[0], a[1], b[1], a[2], b[2]);          // x high
+        // a box spanning the whole range of an axis may touch that axis's slabs
+        const int mx = m.npml[0][0] || m.npml[0][1] ? 1 : 0, my = m.npml[1][0] || m.npml[1][1] ? 2 : 0;
+        add(lo[0], N[0], lo[1], N[1], lo[2], a[2], 4 | mx | my);  // z low
+        add(lo[0], N[0], lo[1], N[1], b[2], N[2], 4 | mx | my);   // z high
+        add(lo[0], N[0], lo[1], a[1], a[2], b[2], 2 | mx);        // y low
+        add(lo[0], N[0], b[1], N[1], a[2], b[2], 2 | mx);         // y high
+        add(lo[0], a[0], a[1], b[1], a[2], b[2], 1);              // x low
+        add(b[0], N[0], a[1], b[1], a[2], b[2], 1);               // x high
+    }
+
+    // Are the stretch tables the lean kernel of one half-step reads constant
+    // (bitwise, in R) over the interior box?  Then their values go in c.
+    bool lean_constant(const Box &L, bool efield, LeanStretch<R> &c) const {
+        if (empty(L)) return false;
+        const int lo[3] = {L.i0, L.j0, L.k0}, hi[3] = {L.i1, L.j1, L.k1};
+        // axis each table is indexed by: cExy, cEyx by k; cEzx, cExz by j; cEzy, cEyz by i
+        static const int axis_of[6] = {2, 2, 1, 1, 0, 0};
+        for (int t = 0; t < 6; t++) {
+            const int tt = t + (efield ? 0 : 6);
+            const std::vector<double> &tab = m.stretch[tt];
+            const int ax = axis_of[t];
+            if ((int)tab.size() < hi[ax]) return false;
+            const R v0 = (R)tab[lo[ax]];
+            for (int q = lo[ax]; q < hi[ax]; q++)
+                if (!((R)tab[q] == v0) || std::signbit((R)tab[q]) != std::signbit(v0)) return false;
+            c.s[tt] = v0;
+        }
+        return true;
     }
 
     void setup_passes() {
         classify(false, leanH, shellH);
         classify(true, leanE, shellE);
-        auto upload_boxes = [&](std::vector<Box> &v, Box *&dev, int &nblk) {
+        auto box_set = [&](std::vector<Box> &v, BoxSet &set, int &nblk) {
+            if (v.size() > (size_t)kMaxBoxes) throw Error("internal: too many shell boxes");
             nblk = 0;
+            set = BoxSet{};
             for (Box &b : v) { b.bstart = nblk; nblk += (int)blocks(b); }
-            if (!v.empty()) {
-                dev = alloc<Box>(v.size());
-                HIP_OK(hipMemcpyAsync(dev, v.data(), v.size() * sizeof(Box), hipMemcpyHostToDevice, stream));
-            }
+            for (size_t q = 0; q < v.size(); q++) set.b[q] = v[q];
+            set.n = (int)v.size();
         };
-        upload_boxes(shellH, shellH_dev, shellH_blocks);
-        upload_boxes(shellE, shellE_dev, shellE_blocks);
-        // interior Drude-Lorentz / gain cells, in device order
+        box_set(shellH, shellH_set, shellH_blocks);
+        box_set(shellE, shellE_set, shellE_blocks);
+        leanH_cs = lean_constant(leanH, false, leanH_st);
+        leanE_cs = lean_constant(leanE, true, leanE_st);
+        // every Drude-Lorentz / gain cell, in device order; cells outside the
+        // interior box carry the CPML flag
         std::vector<long long> sp;
         auto interior = [&](int i, int j, int k) {
             return i >= leanE.i0 && i < leanE.i1 && j >= leanE.j0 && j < leanE.j1 && k >= leanE.k0 && k < leanE.k1;
@@ -398,34 +433,62 @@ public:
         for (const std::vector<int> *cells : {&m.dl_cells, &m.gain_cells})
             for (int f : *cells) {
                 const int i = f / ((m.Ny + 1) * (m.Nz + 1)), j = (f / (m.Nz + 1)) % (m.Ny + 1), k = f % (m.Nz + 1);
-                if (interior(i, j, k)) sp.push_back(dev_index(i, j, k));
+                sp.push_back(dev_index(i, j, k) | (interior(i, j, k) ? 0LL : 1LL << kSpecialPml));
             }
-        std::sort(sp.begin(), sp.end());
+        const long long mask = (1LL << kSpecialPml) - 1;
+        std::sort(sp.begin(), sp.end(), [&](long long a, long long b) { return (a & mask) < (b & mask); });
         nspecial = (int)sp.size();
         if (nspecial) special = upload<long long>(sp);
         HIP_OK(hipStreamSynchronize(stream));
     }
 
     void launch_update(bool h) {
-        if (!split_passes) {
+        if (form == kFormGrid) {
             if (h) launch_h(H_ALL, "update_h");
             else launch_e(E_ALL, "update_e");
             return;
         }
         const Box &L = h ? leanH : leanE;
+        if (form == kFormMerged) {
+            const TileGrid g = tiles();
+            const unsigned nb = (unsigned)(g.gx * g.gy * g.gz);
+            timer.run(stream, h ? "full_h" : "full_e", [&] {
+                if (h) {
+                    if (leanH_cs) hipLaunchKernelGGL((k_full_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanH_st);
+                    else hipLaunchKernelGGL((k_full_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanH_st);
+                } else {
+                    if (leanE_cs) hipLaunchKernelGGL((k_full_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanE_st);
+                    else hipLaunchKernelGGL((k_full_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanE_st);
+                }
+            });
+            HIP_OK(hipGetLastError());
+            if (!h && nspecial) {
+                timer.run(stream, "special_e", [&] {
+                    hipLaunchKernelGGL(k_special_e<R>, dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, special, nspecial);
+                });
+                HIP_OK(hipGetLastError());
+            }
+            return;
+        }
         const long long nl = blocks(L);
         if (nl) {
+            const int skip5 = m.has_gain ? 1 : 0;
             timer.run(stream, h ? "lean_h" : "lean_e", [&] {
-                if (h) hipLaunchKernelGGL(k_lean_h<R>, dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L);
-                else hipLaunchKernelGGL(k_lean_e<R>, dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, m.has_gain ? 1 : 0);
+                if (h) {
+                    if (leanH_cs) hipLaunchKernelGGL((k_lean_h<R, true>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanH_st);
+                    else hipLaunchKernelGGL((k_lean_h<R, false>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanH_st);
+                } else {
+                    if (leanE_cs) hipLaunchKernelGGL((k_lean_e<R, true>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanE_st, skip5);
+                    else hipLaunchKernelGGL((k_lean_e<R, false>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanE_st, skip5);
+                }
             });
             HIP_OK(hipGetLastError());
         }
         const int nb = h ? shellH_blocks : shellE_blocks;
         if (nb) {
             timer.run(stream, h ? "shell_h" : "shell_e", [&] {
-                if (h) hipLaunchKernelGGL(k_shell_h<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellH_dev, (int)shellH.size());
-                else hipLaunchKernelGGL(k_shell_e<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellE_dev, (int)shellE.size());
+                if (h) hipLaunchKernelGGL(k_shell_h<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
+                else hipLaunchKernelGGL(k_shell_e<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
             });
             HIP_OK(hipGetLastError());
         }

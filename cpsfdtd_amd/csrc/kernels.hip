@@ -30,9 +30,10 @@ __device__ __forceinline__ long long cell_index(int px, int py, int i, int j, in
 
 // H at one cell: BasicFDTD.cpp:67-81, then CPML.cpp:609-689 (x, then y, then
 // z slab contributions, in the reference's order).  *_i1/_j1/_k1 are E at the
-// +x/+y/+z neighbour.
+// +x/+y/+z neighbour.  Bit a of pm enables the slab test along axis a (a
+// caller passes 0 for an axis none of its cells can be in).
 template <class R>
-__device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, bool pml, int i, int j, int k, long long p,
+__device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm, int i, int j, int k, long long p,
                                        R ex, R ey, R ez, R ex_j1, R ez_j1, R ex_k1, R ey_k1, R ey_i1, R ez_i1,
                                        R &hx, R &hy, R &hz)
 {
@@ -43,51 +44,54 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, bool pml, in
         hy = aH * hy - bH * (ex_k1 - ex) * d.st[S_cHyx][k] + bH * (ez_i1 - ez) * d.st[S_cHyz][i];
         hz = aH * hz + bH * (ex_j1 - ex) * d.st[S_cHzx][j] - bH * (ey_i1 - ey) * d.st[S_cHzy][i];
     }
-    if (!pml) return;
+    if (!pm) return;
     const R cH = d.cpmlH;
     const bool fixed = d.hsign_fixed;
-    for (int e = 0; e < 2; e++) {  // x slabs: Hy += psi(dEz/dx), Hz -= psi(dEy/dx)
+    for (int e = 0; e < 2 && (pm & 1u); e++) {  // x slabs: Hy += psi(dEz/dx), Hz -= psi(dEy/dx)
         const int n = d.npml[0][e];
         const int l = e == 0 ? i : i - (d.Nx - n);
         if (n && l >= 0 && l < n) {
             const R b = d.pb[1][0][e][l], c = d.pc[1][0][e][l];
             const long long qi = l + (long long)n * (j + (long long)d.py * k);
             R *q1 = d.q[1][0][e][0], *q2 = d.q[1][0][e][1];
-            const R a1 = b * q1[qi] + c * (ez_i1 - ez);
+            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
+            const R a1 = b * q1v + c * (ez_i1 - ez);
+            const R a2 = b * q2v + c * (ey_i1 - ey);
             q1[qi] = a1;
-            hy = hy + a1 * cH;
-            const R a2 = b * q2[qi] + c * (ey_i1 - ey);
             q2[qi] = a2;
+            hy = hy + a1 * cH;
             hz = hz - a2 * cH;
         }
     }
-    for (int e = 0; e < 2; e++) {  // y slabs: Hz -/+ psi(dEx/dy), Hx +/- psi(dEz/dy)
+    for (int e = 0; e < 2 && (pm & 2u); e++) {  // y slabs: Hz -/+ psi(dEx/dy), Hx +/- psi(dEz/dy)
         const int n = d.npml[1][e];
         const int l = e == 0 ? j : j - (d.Ny - n);
         if (n && l >= 0 && l < n) {
             const R b = d.pb[1][1][e][l], c = d.pc[1][1][e][l];
             const long long qi = i + (long long)d.px * (l + (long long)n * k);
             R *q1 = d.q[1][1][e][0], *q2 = d.q[1][1][e][1];
-            const R a1 = b * q1[qi] + c * (ex_j1 - ex);
+            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
+            const R a1 = b * q1v + c * (ex_j1 - ex);
+            const R a2 = b * q2v + c * (ez_j1 - ez);
             q1[qi] = a1;
-            hz = fixed ? hz + a1 * cH : hz - a1 * cH;
-            const R a2 = b * q2[qi] + c * (ez_j1 - ez);
             q2[qi] = a2;
+            hz = fixed ? hz + a1 * cH : hz - a1 * cH;
             hx = fixed ? hx - a2 * cH : hx + a2 * cH;
         }
     }
-    for (int e = 0; e < 2; e++) {  // z slabs: Hx -/+ psi(dEy/dz), Hy +/- psi(dEx/dz)
+    for (int e = 0; e < 2 && (pm & 4u); e++) {  // z slabs: Hx -/+ psi(dEy/dz), Hy +/- psi(dEx/dz)
         const int n = d.npml[2][e];
         const int l = e == 0 ? k : k - (d.Nz - n);
         if (n && l >= 0 && l < n) {
             const R b = d.pb[1][2][e][l], c = d.pc[1][2][e][l];
             const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
             R *q1 = d.q[1][2][e][0], *q2 = d.q[1][2][e][1];
-            const R a1 = b * q1[qi] + c * (ey_k1 - ey);
+            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
+            const R a1 = b * q1v + c * (ey_k1 - ey);
+            const R a2 = b * q2v + c * (ex_k1 - ex);
             q1[qi] = a1;
-            hx = fixed ? hx + a1 * cH : hx - a1 * cH;
-            const R a2 = b * q2[qi] + c * (ex_k1 - ex);
             q2[qi] = a2;
+            hx = fixed ? hx + a1 * cH : hx - a1 * cH;
             hy = fixed ? hy - a2 * cH : hy + a2 * cH;
         }
     }
@@ -97,9 +101,10 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, bool pml, in
 // (ID 4 zeroed, ID 2 left to the ADE), the Drude-Lorentz ADE for ID 2
 // (Drude_Lorentz.cpp:172-206, all three components, C1/C5 per F9/F10), the
 // gain current and FP64 populations for ID 5, then CPML.cpp:529-607.
-// *_im/_jm/_km are H (already updated this step) at the -x/-y/-z neighbour.
+// *_im/_jm/_km are H (already updated this step) at the -x/-y/-z neighbour;
+// pm as for h_cell.
 template <class R>
-__device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, bool pml, int i, int j, int k, long long p,
+__device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned pm, int i, int j, int k, long long p,
                                        int id, R hx, R hy, R hz, R hx_jm, R hz_jm, R hx_km, R hy_km, R hy_im,
                                        R hz_im, R (&E)[3])
 {
@@ -185,9 +190,9 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, bool pml
             d.gN[gs] = d.g_Ntot - n1 - n2 - n3;
         }
     }
-    if (!pml) return;
+    if (!pm) return;
     const R cE = d.cpmlE;
-    for (int e = 0; e < 2; e++) {  // x slabs: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
+    for (int e = 0; e < 2 && (pm & 1u); e++) {  // x slabs: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
         const int n = d.npml[0][e];
         const int l = e == 0 ? i - 1 : i - (d.Nx - n);
         if (n && l >= 0 && l < n) {
@@ -195,15 +200,16 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, bool pml
             const R b = d.pb[0][0][e][ci], c = d.pc[0][0][e][ci];
             const long long qi = l + (long long)n * (j + (long long)d.py * k);
             R *q1 = d.q[0][0][e][0], *q2 = d.q[0][0][e][1];
-            const R a1 = b * q1[qi] + c * (hz - hz_im);
+            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
+            const R a1 = b * q1v + c * (hz - hz_im);
+            const R a2 = b * q2v + c * (hy - hy_im);
             q1[qi] = a1;
-            E[1] = E[1] - a1 * cE;
-            const R a2 = b * q2[qi] + c * (hy - hy_im);
             q2[qi] = a2;
+            E[1] = E[1] - a1 * cE;
             E[2] = E[2] + a2 * cE;
         }
     }
-    for (int e = 0; e < 2; e++) {  // y slabs: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
+    for (int e = 0; e < 2 && (pm & 2u); e++) {  // y slabs: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
         const int n = d.npml[1][e];
         const int l = e == 0 ? j - 1 : j - (d.Ny - n);
         if (n && l >= 0 && l < n) {
@@ -211,15 +217,16 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, bool pml
             const R b = d.pb[0][1][e][cj], c = d.pc[0][1][e][cj];
             const long long qi = i + (long long)d.px * (l + (long long)n * k);
             R *q1 = d.q[0][1][e][0], *q2 = d.q[0][1][e][1];
-            const R a1 = b * q1[qi] + c * (hx - hx_jm);
+            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
+            const R a1 = b * q1v + c * (hx - hx_jm);
+            const R a2 = b * q2v + c * (hz - hz_jm);
             q1[qi] = a1;
-            E[2] = E[2] - a1 * cE;
-            const R a2 = b * q2[qi] + c * (hz - hz_jm);
             q2[qi] = a2;
+            E[2] = E[2] - a1 * cE;
             E[0] = E[0] + a2 * cE;
         }
     }
-    for (int e = 0; e < 2; e++) {  // z slabs: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
+    for (int e = 0; e < 2 && (pm & 4u); e++) {  // z slabs: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
         const int n = d.npml[2][e];
         const int l = e == 0 ? k - 1 : k - (d.Nz - n);
         if (n && l >= 0 && l < n) {
@@ -227,11 +234,12 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, bool pml
             const R b = d.pb[0][2][e][ck], c = d.pc[0][2][e][ck];
             const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
             R *q1 = d.q[0][2][e][0], *q2 = d.q[0][2][e][1];
-            const R a1 = b * q1[qi] + c * (hy - hy_km);
+            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
+            const R a1 = b * q1v + c * (hy - hy_km);
+            const R a2 = b * q2v + c * (hx - hx_km);
             q1[qi] = a1;
-            E[0] = E[0] - a1 * cE;
-            const R a2 = b * q2[qi] + c * (hx - hx_km);
             q2[qi] = a2;
+            E[0] = E[0] - a1 * cE;
             E[1] = E[1] + a2 * cE;
         }
     }
@@ -286,7 +294,7 @@ __global__ __launch_bounds__(TX *TY) void k_update_h(Dev<R> d, unsigned parts, T
     const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
     const long long pk1 = p + d.slab;
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    h_cell(d, (parts & H_BASIC) != 0, pml, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
+    h_cell(d, (parts & H_BASIC) != 0, pml ? 7u : 0u, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
            d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
     d.F[3][p] = hx;
     d.F[4][p] = hy;
@@ -312,7 +320,7 @@ __global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts, T
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
     R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
-    e_cell(d, parts, pml, i, j, k, p, id, d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm], d.F[3][pkm],
+    e_cell(d, parts, pml ? 7u : 0u, i, j, k, p, id, d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm], d.F[3][pkm],
            d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
     d.F[0][p] = E[0];
     d.F[1][p] = E[1];
@@ -334,14 +342,24 @@ struct Box {
     int i0, i1, j0, j1, k0, k1;  // cell ranges [lo, hi)
     int tw, th;                  // tile width (x) and height (y), tw * th = 256
     int xbase, gx, gy, bstart;   // x tiles start at xbase (aligned); block offset
+    int pml;                     // bit a: cells of the box may lie in a CPML slab along axis a
+};
+
+// the shell boxes of one half-step, passed by value (kernel-argument memory)
+constexpr int kMaxBoxes = 6;
+struct BoxSet {
+    Box b[kMaxBoxes];
+    int n;
 };
 
 // locate this workgroup's box, tile and cell; returns false for lanes outside
-__device__ __forceinline__ bool box_cell(const Box *boxes, int nbox, int &i, int &j, int &k, int &bi)
+__device__ __forceinline__ bool box_cell(const BoxSet &S, int &i, int &j, int &k, int &bi)
 {
     int q = 0;
-    while (q + 1 < nbox && (int)blockIdx.x >= boxes[q + 1].bstart) q++;
-    const Box B = boxes[q];
+#pragma unroll
+    for (int s = 1; s < kMaxBoxes; s++)
+        if (s < S.n && (int)blockIdx.x >= S.b[s].bstart) q = s;
+    const Box &B = S.b[q];
     const int t = blockIdx.x - B.bstart;
     const int bx = t % B.gx, by = (t / B.gx) % B.gy;
     k = B.k0 + t / (B.gx * B.gy);
@@ -352,9 +370,23 @@ __device__ __forceinline__ bool box_cell(const Box *boxes, int nbox, int &i, int
     return i >= B.i0 && i < B.i1 && j < B.j1;
 }
 
-// interior H: BasicFDTD.cpp:67-81 only
+// Stretch factors of the interior box.  Outside the CPML slabs every entry of
+// each stretch table the lean kernels read is the same value (the host checks
+// this bit for bit over the box's index ranges); those kernels then take the
+// twelve values as scalars and skip six table loads per cell.
 template <class R>
-__global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B)
+struct LeanStretch {
+    R s[12];
+};
+template <bool CS, class R>
+__device__ __forceinline__ R lean_st(const Dev<R> &d, const LeanStretch<R> &c, int t, int idx)
+{
+    return CS ? c.s[t] : d.st[t][idx];
+}
+
+// interior H: BasicFDTD.cpp:67-81 only
+template <class R, bool CS>
+__global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<R> c)
 {
     const int t = blockIdx.x;
     const int bx = t % B.gx, by = (t / B.gx) % B.gy, k = B.k0 + t / (B.gx * B.gy);
@@ -368,9 +400,9 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B)
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const int id = d.id[p];
     const R aH = d.aH[id], bH = d.bH[id];
-    hx = aH * hx - bH * (d.F[2][pj1] - ez) * d.st[S_cHxz][j] + bH * (d.F[1][pk1] - ey) * d.st[S_cHxy][k];
-    hy = aH * hy - bH * (d.F[0][pk1] - ex) * d.st[S_cHyx][k] + bH * (d.F[2][pi1] - ez) * d.st[S_cHyz][i];
-    hz = aH * hz + bH * (d.F[0][pj1] - ex) * d.st[S_cHzx][j] - bH * (d.F[1][pi1] - ey) * d.st[S_cHzy][i];
+    hx = aH * hx - bH * (d.F[2][pj1] - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (d.F[1][pk1] - ey) * lean_st<CS>(d, c, S_cHxy, k);
+    hy = aH * hy - bH * (d.F[0][pk1] - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (d.F[2][pi1] - ez) * lean_st<CS>(d, c, S_cHyz, i);
+    hz = aH * hz + bH * (d.F[0][pj1] - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (d.F[1][pi1] - ey) * lean_st<CS>(d, c, S_cHzy, i);
     d.F[3][p] = hx;
     d.F[4][p] = hy;
     d.F[5][p] = hz;
@@ -378,87 +410,204 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B)
 
 // interior E: BasicFDTD.cpp:25-45 for plain cells (ID 4 zeroed; ID 2 and, with a
 // gain medium, ID 5 are the special pass's)
-template <class R>
-__global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, int skip5)
+template <class R, bool CS>
+__global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<R> c, int skip5)
 {
     const int t = blockIdx.x;
     const int bx = t % B.gx, by = (t / B.gx) % B.gy, k = B.k0 + t / (B.gx * B.gy);
     const int i = B.xbase + bx * TX + threadIdx.x, j = B.j0 + by * TY + threadIdx.y;
     if (i < B.i0 || i >= B.i1 || j >= B.j1) return;
     const long long p = cell_index(d.px, d.py, i, j, k);
-    const int id = d.id[p];
-    if (id == 2 || (skip5 && id == 5)) return;
-    if (id == 4) {
-        d.F[0][p] = 0;
-        d.F[1][p] = 0;
-        d.F[2][p] = 0;
-        return;
-    }
     const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
+    // Branch-free on the material: every value is used on every path, so the
+    // field loads issue together with the ID load instead of after it (a
+    // branch on the ID lets the compiler sink them past a wait).  Cells of the
+    // special pass store their own value back unchanged.
+    const int id = d.id[p];
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
+    const R hz_jm = d.F[5][pjm], hy_km = d.F[4][pkm], hx_km = d.F[3][pkm];
+    const R hz_im = d.F[5][pim], hy_im = d.F[4][pim], hx_jm = d.F[3][pjm];
     const R aE = d.aE[id], bE = d.bE[id];
-    const R e0 = aE * d.F[0][p] + bE * (hz - d.F[5][pjm]) * d.st[S_cExz][j] - bE * (hy - d.F[4][pkm]) * d.st[S_cExy][k];
-    const R e1 = aE * d.F[1][p] + bE * (hx - d.F[3][pkm]) * d.st[S_cEyx][k] - bE * (hz - d.F[5][pim]) * d.st[S_cEyz][i];
-    const R e2 = aE * d.F[2][p] + bE * (hy - d.F[4][pim]) * d.st[S_cEzy][i] - bE * (hx - d.F[3][pjm]) * d.st[S_cEzx][j];
+    R e0 = aE * ex + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
+    R e1 = aE * ey + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
+    R e2 = aE * ez + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
+    const bool keep = id == 2 || (skip5 && id == 5);
+    e0 = keep ? ex : id == 4 ? (R)0 : e0;
+    e1 = keep ? ey : id == 4 ? (R)0 : e1;
+    e2 = keep ? ez : id == 4 ? (R)0 : e2;
     d.F[0][p] = e0;
     d.F[1][p] = e1;
     d.F[2][p] = e2;
 }
 
-// CPML slab boxes: full physics
+// CPML slab boxes: full physics, slab tests limited to the box's axes
 template <class R>
-__global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, const Box *boxes, int nbox)
+__global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, BoxSet S)
 {
     int i, j, k, bi;
-    if (!box_cell(boxes, nbox, i, j, k, bi)) return;
+    if (!box_cell(S, i, j, k, bi)) return;
     const long long p = cell_index(d.px, d.py, i, j, k);
     const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
     const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
     const long long pk1 = p + d.slab;
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    h_cell(d, true, true, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1], d.F[0][pk1],
-           d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
+    h_cell(d, true, (unsigned)S.b[bi].pml, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
+           d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
     d.F[3][p] = hx;
     d.F[4][p] = hy;
     d.F[5][p] = hz;
 }
 
 template <class R>
-__device__ __forceinline__ void e_full(const Dev<R> &d, int i, int j, int k, bool pml)
+__device__ __forceinline__ void e_full(const Dev<R> &d, int i, int j, int k, unsigned pm)
 {
     const long long p = cell_index(d.px, d.py, i, j, k);
     const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
     R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
-    e_cell(d, E_ALL, pml, i, j, k, p, d.id[p], d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm],
+    e_cell(d, E_ALL, pm, i, j, k, p, d.id[p], d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm],
            d.F[3][pkm], d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
     d.F[0][p] = E[0];
     d.F[1][p] = E[1];
     d.F[2][p] = E[2];
 }
 
+// CPML shell, E: stencil and psi only.  Drude-Lorentz and gain cells of the
+// shell belong to the special pass, which also runs their psi updates.
 template <class R>
-__global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, const Box *boxes, int nbox)
+__global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
 {
     int i, j, k, bi;
-    if (!box_cell(boxes, nbox, i, j, k, bi)) return;
-    e_full(d, i, j, k, true);
+    if (!box_cell(S, i, j, k, bi)) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
+    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
+    const long long pkm = p - d.slab;
+    const int id = d.id[p];
+    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
+    const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
+    // special-pass cells: no psi update here, own values stored back (no
+    // branch on the ID ahead of the loads, as in k_lean_e)
+    const bool sp = (id == 2 && d.ndl) || (id == 5 && d.ngain);
+    const R E0[3] = {E[0], E[1], E[2]};
+    e_cell(d, E_BASIC | E_PML, sp ? 0u : (unsigned)S.b[bi].pml, i, j, k, p, id, hx, hy, hz, hx_jm, hz_jm, hx_km,
+           hy_km, hy_im, hz_im, E);
+    for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
+    d.F[0][p] = E[0];
+    d.F[1][p] = E[1];
+    d.F[2][p] = E[2];
 }
 
-// interior Drude-Lorentz and gain cells (device cell indices)
+// ---------------------------------------------------------------------------
+// Merged half-step kernels: one launch over every tile of the update range.
+// A tile inside the interior box L runs the lean stencil; any other tile runs
+// the stencil with CPML terms, testing only the axes whose slabs the tile
+// reaches (workgroup-uniform mask).  Drude-Lorentz / gain cells are left to
+// the special pass (they store their own value back).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool tile_inside(const Box &L, int x0, int y0, int k)
+{
+    return x0 >= L.i0 && x0 + TX <= L.i1 && y0 >= L.j0 && y0 + TY <= L.j1 && k >= L.k0 && k < L.k1;
+}
+template <class R>
+__device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, int x0, int y0, int k, bool efield)
+{
+    const int o = efield ? 1 : 0;  // E slabs start at index 1 on the low side
+    unsigned m = 0;
+    if ((d.npml[0][0] && x0 < d.npml[0][0] + o) || (d.npml[0][1] && x0 + TX > d.Nx - d.npml[0][1])) m |= 1u;
+    if ((d.npml[1][0] && y0 < d.npml[1][0] + o) || (d.npml[1][1] && y0 + TY > d.Ny - d.npml[1][1])) m |= 2u;
+    if ((d.npml[2][0] && k < d.npml[2][0] + o) || (d.npml[2][1] && k >= d.Nz - d.npml[2][1])) m |= 4u;
+    return m;
+}
+
+template <class R, bool CS>
+__global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d, TileGrid g, Box L, LeanStretch<R> c)
+{
+    int bx, by, k;
+    if (!tile_of(g, bx, by, k)) return;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+    if (i >= d.Nx || j >= d.Ny) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
+    const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
+    const long long pk1 = p + d.slab;
+    const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
+    const R ex_j1 = d.F[0][pj1], ez_j1 = d.F[2][pj1], ex_k1 = d.F[0][pk1];
+    const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
+    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    if (tile_inside(L, x0, y0, k)) {
+        const int id = d.id[p];
+        const R aH = d.aH[id], bH = d.bH[id];
+        hx = aH * hx - bH * (ez_j1 - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (ey_k1 - ey) * lean_st<CS>(d, c, S_cHxy, k);
+        hy = aH * hy - bH * (ex_k1 - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (ez_i1 - ez) * lean_st<CS>(d, c, S_cHyz, i);
+        hz = aH * hz + bH * (ex_j1 - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (ey_i1 - ey) * lean_st<CS>(d, c, S_cHzy, i);
+    } else {
+        h_cell(d, true, d.has_pml ? tile_pml_mask(d, x0, y0, k, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
+               ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz);
+    }
+    d.F[3][p] = hx;
+    d.F[4][p] = hy;
+    d.F[5][p] = hz;
+}
+
+template <class R, bool CS>
+__global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d, TileGrid g, Box L, LeanStretch<R> c)
+{
+    int bx, by, k;
+    if (!tile_of(g, bx, by, k)) return;
+    const int x0 = bx * TX, y0 = by * TY;
+    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+    if (i < d.ie0 || i >= d.Nx || j < d.je0 || j >= d.Ny || k < 1 || k >= d.Nz) return;
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
+    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
+    const long long pkm = p - d.slab;
+    const int id = d.id[p];
+    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
+    const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
+    const bool sp = (id == 2 && d.ndl) || (id == 5 && d.ngain);  // the special pass's cells
+    const R E0[3] = {E[0], E[1], E[2]};
+    if (tile_inside(L, x0, y0, k)) {
+        const R aE = d.aE[id], bE = d.bE[id];
+        E[0] = aE * E[0] + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
+        E[1] = aE * E[1] + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
+        E[2] = aE * E[2] + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
+        const bool keep = sp || id == 2;  // ID 2 without the ADE module is left as is
+        for (int q = 0; q < 3; q++) E[q] = keep ? E0[q] : id == 4 ? (R)0 : E[q];
+    } else {
+        e_cell(d, E_BASIC | E_PML, sp || !d.has_pml ? 0u : tile_pml_mask(d, x0, y0, k, true), i, j, k, p, id, hx, hy,
+               hz, hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E);
+        for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
+    }
+    d.F[0][p] = E[0];
+    d.F[1][p] = E[1];
+    d.F[2][p] = E[2];
+}
+
+// Drude-Lorentz and gain cells, all of them: device cell index, with bit
+// kSpecialPml set for a cell outside the interior box (its CPML slab terms
+// follow the ADE, as in the full update).
+constexpr int kSpecialPml = 62;
 template <class R>
 __global__ __launch_bounds__(256) void k_special_e(Dev<R> d, const long long *cells, int n)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
-    const long long p = cells[t];
+    const long long v = cells[t];
+    const long long p = v & ((1LL << kSpecialPml) - 1);
     const int k = (int)(p / d.slab);
     const long long r = p - (long long)k * d.slab;
     const int j = (int)(r / d.px), i = (int)(r - (long long)j * d.px);
-    e_full(d, i, j, k, false);
+    e_full(d, i, j, k, (v >> kSpecialPml) & 1 ? 7u : 0u);
 }
 
 // TF_SF.cpp:966-986

@@ -33,7 +33,10 @@ def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZER
         Shape(5 if gain else 1, "cuboid", x0=250e-9 * sx, y0=230e-9 * sy, z0=200e-9 * sz, a=70e-9, b=60e-9, c=50e-9),
         Shape(4, "multi", theta=2, phi=2, psi=1, x0=100e-9 * sx, y0=100e-9 * sy, z0=200e-9 * sz, a=150e-9 * sx,
               b=120e-9 * sy, c=0.0),
-    ])
+    ] + ([  # dispersive / gain blocks reaching into CPML corners
+        Shape(2, "cuboid", x0=40e-9, y0=180e-9 * sy, z0=40e-9, a=60e-9, b=50e-9, c=60e-9)] if dl else []) + ([
+        Shape(5, "cuboid", x0=(n[0] - 4) * 10e-9, y0=40e-9, z0=(n[2] - 4) * 10e-9, a=60e-9, b=60e-9, c=60e-9)]
+        if gain else []))
     if npml:
         w.cpml(((npml[0], npml[0]), (npml[1], npml[1]), (npml[2], npml[2])), m=3.0, malp=1.0, kapmax=5.0,
                sig_ratio=1.0, alpmax=0.05)

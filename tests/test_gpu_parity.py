@@ -73,9 +73,10 @@ def test_mixed_f64_bit_exact(tmp_path, module, flags):
         assert rel_l2(e.port_intensity(q), o.port_intensity(q)) < 1e-12
 
 
-@pytest.mark.parametrize("form", ["split", "grid"])
+@pytest.mark.parametrize("form", ["merged", "split", "grid"])
 def test_update_forms_bit_exact(tmp_path, monkeypatch, form):
-    """Both fused-step forms: lean/shell/special passes and the single
+    """Every fused-step form: merged (one launch per half-step, lean or CPML
+    path per tile), split (lean box, shell boxes, special cells) and the single
     full-physics grid kernel per half-step."""
     monkeypatch.setenv("CFD_UPDATE_KERNEL", form)
     case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
