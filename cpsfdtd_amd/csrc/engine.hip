@@ -165,6 +165,9 @@ public:
     }
 
     void setup() {
+        // measured on MI355X (256^3, 12-cell CPML): merged is faster in FP32;
+        // in FP64 its larger register footprint costs occupancy and split wins
+        form = sizeof(R) == 4 ? kFormMerged : kFormSplit;
         if (const char *v = std::getenv("CFD_UPDATE_KERNEL")) {
             const std::string f(v);
             form = f == "grid" ? kFormGrid : f == "merged" ? kFormMerged : kFormSplit;
@@ -333,13 +336,15 @@ public:
         return g;
     }
     // Fused-step update forms (CFD_UPDATE_KERNEL selects one, for A/B checks):
-    //   split (default)   lean interior box, CPML shell boxes, special cells
+    //   split             lean interior box, CPML shell boxes, special cells
+    //                     (default in FP64)
     //   merged            one launch per half-step over all tiles, lean stencil
     //                     on interior tiles, CPML terms on the others, plus the
-    //                     special pass for Drude-Lorentz / gain cells
+    //                     special pass for Drude-Lorentz / gain cells (default
+    //                     in FP32)
     //   grid              the single full-physics grid kernel per half-step
     enum { kFormMerged, kFormSplit, kFormGrid };
-    int form = kFormSplit;
+    int form = kFormMerged;
     Box leanH{}, leanE{};
     std::vector<Box> shellH, shellE;
     BoxSet shellH_set{}, shellE_set{};

@@ -28,6 +28,66 @@ __device__ __forceinline__ long long cell_index(int px, int py, int i, int j, in
 // z-marching kernels (fused step).  Neighbour values arrive as arguments.
 // ---------------------------------------------------------------------------
 
+// CPML term of one axis at one cell.  The two slabs of an axis never overlap
+// (the model rejects n_low + n_high >= N), so at most one holds the cell.
+template <class R>
+struct SlabTerm {
+    R b, c, q1, q2;  // grading coefficients and the two psi values
+    long long qi;    // psi index in the slab box
+    int side;        // -1: outside both slabs; 0: low side; 1: high side
+};
+
+// Locate the slab of each enabled axis (bit a of pm) holding cell (i, j, k)
+// for field f (0: E, 1: H) and load its coefficients and both psi values, so
+// the loads issue together with the caller's field loads.  E slabs start at
+// index 1 on the low side, whose coefficient tables are indexed by the cell
+// position (CPML.cpp:529-607); H slabs start at 0 (CPML.cpp:609-689).
+template <class R>
+__device__ __forceinline__ void gather_slabs(const Dev<R> &d, int f, unsigned pm, int i, int j, int k,
+                                             SlabTerm<R> (&T)[3])
+{
+    const int pos[3] = {i, j, k};
+    const int N[3] = {d.Nx, d.Ny, d.Nz};
+    const int o = f == 0 ? 1 : 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        T[ax].side = -1;
+        if (!(pm & (1u << ax))) continue;
+        const int n0 = d.npml[ax][0], n1 = d.npml[ax][1];
+        const int x = pos[ax];
+        int side = -1, l = 0, ci = 0;
+        if (n0 && x - o >= 0 && x - o < n0) {
+            side = 0; l = x - o; ci = x;
+        } else if (n1 && x >= N[ax] - n1 && x < N[ax]) {
+            side = 1; l = x - (N[ax] - n1); ci = l;
+        }
+        if (side < 0) continue;
+        const int n = side ? n1 : n0;
+        const long long qi = ax == 0 ? l + (long long)n * (j + (long long)d.py * k)
+                           : ax == 1 ? i + (long long)d.px * (l + (long long)n * k)
+                                     : i + (long long)d.px * (j + (long long)d.py * l);
+        const R *pb = side ? d.pb[f][ax][1] : d.pb[f][ax][0];
+        const R *pc = side ? d.pc[f][ax][1] : d.pc[f][ax][0];
+        const R *q1 = side ? d.q[f][ax][1][0] : d.q[f][ax][0][0];
+        const R *q2 = side ? d.q[f][ax][1][1] : d.q[f][ax][0][1];
+        T[ax].b = pb[ci];
+        T[ax].c = pc[ci];
+        T[ax].q1 = q1[qi];
+        T[ax].q2 = q2[qi];
+        T[ax].qi = qi;
+        T[ax].side = side;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void slab_store(const Dev<R> &d, int f, int ax, const SlabTerm<R> &t, R a1, R a2)
+{
+    R *q1 = t.side ? d.q[f][ax][1][0] : d.q[f][ax][0][0];
+    R *q2 = t.side ? d.q[f][ax][1][1] : d.q[f][ax][0][1];
+    q1[t.qi] = a1;
+    q2[t.qi] = a2;
+}
+
 // H at one cell: BasicFDTD.cpp:67-81, then CPML.cpp:609-689 (x, then y, then
 // z slab contributions, in the reference's order).  *_i1/_j1/_k1 are E at the
 // +x/+y/+z neighbour.  Bit a of pm enables the slab test along axis a (a
@@ -37,63 +97,45 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
                                        R ex, R ey, R ez, R ex_j1, R ez_j1, R ex_k1, R ey_k1, R ey_i1, R ez_i1,
                                        R &hx, R &hy, R &hz)
 {
+    // ID and stretch loads first, then the slab terms: every load of the cell
+    // is in flight before the first wait
+    const int id = basic ? (int)d.id[p] : 0;
+    R sxz = 0, sxy = 0, syx = 0, syz = 0, szx = 0, szy = 0;
     if (basic) {
-        const int id = d.id[p];
+        sxz = d.st[S_cHxz][j]; sxy = d.st[S_cHxy][k]; syx = d.st[S_cHyx][k];
+        syz = d.st[S_cHyz][i]; szx = d.st[S_cHzx][j]; szy = d.st[S_cHzy][i];
+    }
+    SlabTerm<R> T[3];
+    gather_slabs(d, 1, pm, i, j, k, T);
+    if (basic) {
         const R aH = d.aH[id], bH = d.bH[id];
-        hx = aH * hx - bH * (ez_j1 - ez) * d.st[S_cHxz][j] + bH * (ey_k1 - ey) * d.st[S_cHxy][k];
-        hy = aH * hy - bH * (ex_k1 - ex) * d.st[S_cHyx][k] + bH * (ez_i1 - ez) * d.st[S_cHyz][i];
-        hz = aH * hz + bH * (ex_j1 - ex) * d.st[S_cHzx][j] - bH * (ey_i1 - ey) * d.st[S_cHzy][i];
+        hx = aH * hx - bH * (ez_j1 - ez) * sxz + bH * (ey_k1 - ey) * sxy;
+        hy = aH * hy - bH * (ex_k1 - ex) * syx + bH * (ez_i1 - ez) * syz;
+        hz = aH * hz + bH * (ex_j1 - ex) * szx - bH * (ey_i1 - ey) * szy;
     }
     if (!pm) return;
     const R cH = d.cpmlH;
     const bool fixed = d.hsign_fixed;
-    for (int e = 0; e < 2 && (pm & 1u); e++) {  // x slabs: Hy += psi(dEz/dx), Hz -= psi(dEy/dx)
-        const int n = d.npml[0][e];
-        const int l = e == 0 ? i : i - (d.Nx - n);
-        if (n && l >= 0 && l < n) {
-            const R b = d.pb[1][0][e][l], c = d.pc[1][0][e][l];
-            const long long qi = l + (long long)n * (j + (long long)d.py * k);
-            R *q1 = d.q[1][0][e][0], *q2 = d.q[1][0][e][1];
-            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
-            const R a1 = b * q1v + c * (ez_i1 - ez);
-            const R a2 = b * q2v + c * (ey_i1 - ey);
-            q1[qi] = a1;
-            q2[qi] = a2;
-            hy = hy + a1 * cH;
-            hz = hz - a2 * cH;
-        }
+    if (T[0].side >= 0) {  // x slab: Hy += psi(dEz/dx), Hz -= psi(dEy/dx)
+        const R a1 = T[0].b * T[0].q1 + T[0].c * (ez_i1 - ez);
+        const R a2 = T[0].b * T[0].q2 + T[0].c * (ey_i1 - ey);
+        slab_store(d, 1, 0, T[0], a1, a2);
+        hy = hy + a1 * cH;
+        hz = hz - a2 * cH;
     }
-    for (int e = 0; e < 2 && (pm & 2u); e++) {  // y slabs: Hz -/+ psi(dEx/dy), Hx +/- psi(dEz/dy)
-        const int n = d.npml[1][e];
-        const int l = e == 0 ? j : j - (d.Ny - n);
-        if (n && l >= 0 && l < n) {
-            const R b = d.pb[1][1][e][l], c = d.pc[1][1][e][l];
-            const long long qi = i + (long long)d.px * (l + (long long)n * k);
-            R *q1 = d.q[1][1][e][0], *q2 = d.q[1][1][e][1];
-            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
-            const R a1 = b * q1v + c * (ex_j1 - ex);
-            const R a2 = b * q2v + c * (ez_j1 - ez);
-            q1[qi] = a1;
-            q2[qi] = a2;
-            hz = fixed ? hz + a1 * cH : hz - a1 * cH;
-            hx = fixed ? hx - a2 * cH : hx + a2 * cH;
-        }
+    if (T[1].side >= 0) {  // y slab: Hz -/+ psi(dEx/dy), Hx +/- psi(dEz/dy)
+        const R a1 = T[1].b * T[1].q1 + T[1].c * (ex_j1 - ex);
+        const R a2 = T[1].b * T[1].q2 + T[1].c * (ez_j1 - ez);
+        slab_store(d, 1, 1, T[1], a1, a2);
+        hz = fixed ? hz + a1 * cH : hz - a1 * cH;
+        hx = fixed ? hx - a2 * cH : hx + a2 * cH;
     }
-    for (int e = 0; e < 2 && (pm & 4u); e++) {  // z slabs: Hx -/+ psi(dEy/dz), Hy +/- psi(dEx/dz)
-        const int n = d.npml[2][e];
-        const int l = e == 0 ? k : k - (d.Nz - n);
-        if (n && l >= 0 && l < n) {
-            const R b = d.pb[1][2][e][l], c = d.pc[1][2][e][l];
-            const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
-            R *q1 = d.q[1][2][e][0], *q2 = d.q[1][2][e][1];
-            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
-            const R a1 = b * q1v + c * (ey_k1 - ey);
-            const R a2 = b * q2v + c * (ex_k1 - ex);
-            q1[qi] = a1;
-            q2[qi] = a2;
-            hx = fixed ? hx + a1 * cH : hx - a1 * cH;
-            hy = fixed ? hy - a2 * cH : hy + a2 * cH;
-        }
+    if (T[2].side >= 0) {  // z slab: Hx -/+ psi(dEy/dz), Hy +/- psi(dEx/dz)
+        const R a1 = T[2].b * T[2].q1 + T[2].c * (ey_k1 - ey);
+        const R a2 = T[2].b * T[2].q2 + T[2].c * (ex_k1 - ex);
+        slab_store(d, 1, 2, T[2], a1, a2);
+        hx = fixed ? hx + a1 * cH : hx - a1 * cH;
+        hy = fixed ? hy - a2 * cH : hy + a2 * cH;
     }
 }
 
@@ -108,6 +150,14 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
                                        int id, R hx, R hy, R hz, R hx_jm, R hz_jm, R hx_km, R hy_km, R hy_im,
                                        R hz_im, R (&E)[3])
 {
+    // stretch values and slab terms loaded ahead of every store below
+    R sxz = 0, sxy = 0, syx = 0, syz = 0, szy = 0, szx = 0;
+    if (parts & (E_BASIC | E_DL)) {
+        sxz = d.st[S_cExz][j]; sxy = d.st[S_cExy][k]; syx = d.st[S_cEyx][k];
+        syz = d.st[S_cEyz][i]; szy = d.st[S_cEzy][i]; szx = d.st[S_cEzx][j];
+    }
+    SlabTerm<R> T[3];
+    gather_slabs(d, 0, pm, i, j, k, T);
     const unsigned gparts = parts & (E_GAINPOL | E_GAINE | E_GAINPOP);
     const bool gain = gparts && d.ngain && id == 5;
     const bool dl = (parts & E_DL) && d.ndl && id == 2;
@@ -132,17 +182,17 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
             E[0] = 0; E[1] = 0; E[2] = 0;
         } else if (id != 2) {
             const R aE = d.aE[id], bE = d.bE[id];
-            E[0] = aE * E[0] + bE * (hz - hz_jm) * d.st[S_cExz][j] - bE * (hy - hy_km) * d.st[S_cExy][k];
-            E[1] = aE * E[1] + bE * (hx - hx_km) * d.st[S_cEyx][k] - bE * (hz - hz_im) * d.st[S_cEyz][i];
-            E[2] = aE * E[2] + bE * (hy - hy_im) * d.st[S_cEzy][i] - bE * (hx - hx_jm) * d.st[S_cEzx][j];
+            E[0] = aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy;
+            E[1] = aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz;
+            E[2] = aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx;
         }
     }
     if (dl) {
         const int s = d.aux[p];
         const R dA[3] = {hz - hz_jm, hx - hx_km, hy - hy_im};
-        const R cA[3] = {d.st[S_cExz][j], d.st[S_cEyx][k], d.st[S_cEzy][i]};
+        const R cA[3] = {sxz, syx, szy};
         const R dB[3] = {hy - hy_km, hz - hz_im, hx - hx_jm};
-        const R cB[3] = {d.st[S_cExy][k], d.st[S_cEyz][i], d.st[S_cEzx][j]};
+        const R cB[3] = {sxy, syz, szx};
         const int nD = d.nD, nL = d.nL;
         const long long ndl = d.ndl, vs = 3 * ndl;
         for (int c = 0; c < 3; c++) {
@@ -192,56 +242,26 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
     }
     if (!pm) return;
     const R cE = d.cpmlE;
-    for (int e = 0; e < 2 && (pm & 1u); e++) {  // x slabs: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
-        const int n = d.npml[0][e];
-        const int l = e == 0 ? i - 1 : i - (d.Nx - n);
-        if (n && l >= 0 && l < n) {
-            const int ci = e == 0 ? i : l;
-            const R b = d.pb[0][0][e][ci], c = d.pc[0][0][e][ci];
-            const long long qi = l + (long long)n * (j + (long long)d.py * k);
-            R *q1 = d.q[0][0][e][0], *q2 = d.q[0][0][e][1];
-            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
-            const R a1 = b * q1v + c * (hz - hz_im);
-            const R a2 = b * q2v + c * (hy - hy_im);
-            q1[qi] = a1;
-            q2[qi] = a2;
-            E[1] = E[1] - a1 * cE;
-            E[2] = E[2] + a2 * cE;
-        }
+    if (T[0].side >= 0) {  // x slab: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
+        const R a1 = T[0].b * T[0].q1 + T[0].c * (hz - hz_im);
+        const R a2 = T[0].b * T[0].q2 + T[0].c * (hy - hy_im);
+        slab_store(d, 0, 0, T[0], a1, a2);
+        E[1] = E[1] - a1 * cE;
+        E[2] = E[2] + a2 * cE;
     }
-    for (int e = 0; e < 2 && (pm & 2u); e++) {  // y slabs: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
-        const int n = d.npml[1][e];
-        const int l = e == 0 ? j - 1 : j - (d.Ny - n);
-        if (n && l >= 0 && l < n) {
-            const int cj = e == 0 ? j : l;
-            const R b = d.pb[0][1][e][cj], c = d.pc[0][1][e][cj];
-            const long long qi = i + (long long)d.px * (l + (long long)n * k);
-            R *q1 = d.q[0][1][e][0], *q2 = d.q[0][1][e][1];
-            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
-            const R a1 = b * q1v + c * (hx - hx_jm);
-            const R a2 = b * q2v + c * (hz - hz_jm);
-            q1[qi] = a1;
-            q2[qi] = a2;
-            E[2] = E[2] - a1 * cE;
-            E[0] = E[0] + a2 * cE;
-        }
+    if (T[1].side >= 0) {  // y slab: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
+        const R a1 = T[1].b * T[1].q1 + T[1].c * (hx - hx_jm);
+        const R a2 = T[1].b * T[1].q2 + T[1].c * (hz - hz_jm);
+        slab_store(d, 0, 1, T[1], a1, a2);
+        E[2] = E[2] - a1 * cE;
+        E[0] = E[0] + a2 * cE;
     }
-    for (int e = 0; e < 2 && (pm & 4u); e++) {  // z slabs: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
-        const int n = d.npml[2][e];
-        const int l = e == 0 ? k - 1 : k - (d.Nz - n);
-        if (n && l >= 0 && l < n) {
-            const int ck = e == 0 ? k : l;
-            const R b = d.pb[0][2][e][ck], c = d.pc[0][2][e][ck];
-            const long long qi = i + (long long)d.px * (j + (long long)d.py * l);
-            R *q1 = d.q[0][2][e][0], *q2 = d.q[0][2][e][1];
-            const R q1v = q1[qi], q2v = q2[qi];  // loads ahead of both stores (q1, q2 not known distinct)
-            const R a1 = b * q1v + c * (hy - hy_km);
-            const R a2 = b * q2v + c * (hx - hx_km);
-            q1[qi] = a1;
-            q2[qi] = a2;
-            E[0] = E[0] - a1 * cE;
-            E[1] = E[1] + a2 * cE;
-        }
+    if (T[2].side >= 0) {  // z slab: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
+        const R a1 = T[2].b * T[2].q1 + T[2].c * (hy - hy_km);
+        const R a2 = T[2].b * T[2].q2 + T[2].c * (hx - hx_km);
+        slab_store(d, 0, 2, T[2], a1, a2);
+        E[0] = E[0] - a1 * cE;
+        E[1] = E[1] + a2 * cE;
     }
 }
 
