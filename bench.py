@@ -81,6 +81,20 @@ def algorithmic_bytes(geo, S, nd_vars):
     }
 
 
+def profiled_traffic(kernel, path=os.path.join(ROOT, "profiles", "hbm_traffic_c2_256_fp32.txt")):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                f = line.split()
+                if len(f) == 3 and f[0] == "k_" + kernel:
+                    return int(f[1]) + int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def run_engine(case, precision, steps, warmup, timing_steps):
     import cpsfdtd_amd as eng
     e = eng.Engine(case, precision)
@@ -100,7 +114,7 @@ def run_engine(case, precision, steps, warmup, timing_steps):
     return wall, kt, nbytes
 
 
-def cpu_baseline(scale_steps=20):
+def cpu_baseline(scale_steps=200):
     """The reference's own OpenMP loop (oracle/_ref/refdrv, compiled from the
     reference sources) on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -218,7 +232,8 @@ def main():
                                "2 plane ports x 64-frequency FP64 DFT (stride 8), FP32 fields",
                    "cells": cells, "steps_in_run": nt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r32["dom_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r32["dom_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(r32["dom_gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(r32["dominant"]),
+                     "traffic_source": "profiles/hbm_traffic_c2_256_fp32.txt",
                      "kernel": r32["dominant"], "bytes_per_launch": r32["dom_bytes"],
                      "avg_ms": round(r32["kernels"][r32["dominant"]]["avg_ms"], 5)},
         "kernels_f32": {k: round(v["avg_ms"], 5) for k, v in r32["kernels"].items()},

@@ -70,6 +70,7 @@ SIGNATURES = {
     "cfd_port_npts": ([_vp, C.c_int], C.c_int),
     "cfd_port_points": ([_vp, C.c_int, _ip], C.c_int),
     "cfd_port_samples": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_port_gather": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_port_dft": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_port_intensity": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_gain_read": ([_vp, _dp], C.c_int),
@@ -252,6 +253,13 @@ class Engine:
         n = _check(library().cfd_port_npts(self.h, q))
         a = np.zeros((self.steps_done, n, 6), np.float64)
         _check(library().cfd_port_samples(self.h, q, a.ctypes.data_as(_dp)))
+        return a
+
+    def port_gather(self, q):
+        """Current field values at port q's points, [points][6]."""
+        n = _check(library().cfd_port_npts(self.h, q))
+        a = np.zeros((n, 6), np.float64)
+        _check(library().cfd_port_gather(self.h, q, a.ctypes.data_as(_dp)))
         return a
 
     def port_dft(self, q):

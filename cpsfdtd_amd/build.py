@@ -43,6 +43,7 @@ def build(verbose=False, force=True):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [os.path.join(CSRC, f) for f in HOST_SOURCES + HIP_SOURCES + HEADERS]
     srcs.append(os.path.join(ROOT, "include", "cpsfdtd.h"))
+    srcs.append(os.path.join(ROOT, "include", "cpsfdtd.hpp"))
     if not force and os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(s) for s in srcs):
         return LIB
     objs = []
@@ -57,7 +58,23 @@ def build(verbose=False, force=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
+    build_facade_driver(verbose)
     return LIB
+
+
+FACADE_DRIVER_SRC = os.path.join(ROOT, "tests", "facade", "main_loop.cpp")
+FACADE_DRIVER = os.path.join(ROOT, "tests", "facade", "_bin", "main_loop")
+
+
+def build_facade_driver(verbose=False):
+    """The main.cpp-shaped driver written against include/cpsfdtd.hpp, linked
+    to the in-tree library (used by the facade tests)."""
+    if not os.path.exists(FACADE_DRIVER_SRC):
+        return None
+    os.makedirs(os.path.dirname(FACADE_DRIVER), exist_ok=True)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), FACADE_DRIVER_SRC,
+          "-o", FACADE_DRIVER, "-L" + LIBDIR, "-lcpsfdtd", "-Wl,-rpath,$ORIGIN/../../../cpsfdtd_amd/lib"], verbose)
+    return FACADE_DRIVER
 
 
 if __name__ == "__main__":

@@ -95,6 +95,7 @@ public:
     virtual void read_field(int comp, double *host) = 0;
     virtual void write_field(int comp, const double *host) = 0;
     virtual void port_samples(int q, double *host) = 0;
+    virtual void port_gather(int q, double *host) = 0;
     virtual void port_dft(int q, double *host) = 0;
     virtual void port_intensity(int q, double *host) = 0;
     virtual void gain_read(double *host) = 0;
@@ -116,6 +117,8 @@ public:
     // ports
     std::vector<int> dl_slot, gain_slot;  // reference-order cell -> device slot
     std::vector<PortDesc<R>> dports;  // host copy of the port table
+    R *gather_buf = nullptr;          // [point][6] for cfd_port_gather
+    int gather_max = 0;
     PortDesc<R> *port_tab = nullptr;
     int port_blocks = 0;
     static constexpr int kDftBatch = 16;  // sampled steps buffered per DFT pass
@@ -319,7 +322,9 @@ public:
             dp.dft = m.freqs.empty() ? nullptr : alloc<double>((size_t)dp.npts * m.freqs.size() * 12);
             dp.ipart = alloc<double>((size_t)m.nt * std::max(dp.nblk, 1));
             dports.push_back(dp);
+            gather_max = std::max(gather_max, dp.npts);
         }
+        if (gather_max) gather_buf = alloc<R>((size_t)gather_max * 6);
         setup_passes();
         if (!dports.empty()) {
             port_tab = alloc<PortDesc<R>>(dports.size());
@@ -608,6 +613,16 @@ public:
         HIP_OK(hipStreamSynchronize(stream));
         for (size_t t = 0; t < n; t++) host[t] = (double)tmp[t];
     }
+    void port_gather(int q, double *host) override {
+        const PortDesc<R> &p = dports.at(q);
+        if (!p.npts) return;
+        hipLaunchKernelGGL(k_gather<R>, dim3((p.npts + 255) / 256), dim3(256), 0, stream, d, p.cell, p.npts, gather_buf);
+        HIP_OK(hipGetLastError());
+        std::vector<R> tmp((size_t)p.npts * 6);
+        HIP_OK(hipMemcpyAsync(tmp.data(), gather_buf, tmp.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (size_t t = 0; t < tmp.size(); t++) host[t] = (double)tmp[t];
+    }
     void port_dft(int q, double *host) override {
         const PortDesc<R> &p = dports.at(q);
         if (!p.dft) throw Error("no DFT frequencies configured");
@@ -788,6 +803,12 @@ int cfd_port_points(cfd_sim *s, int port, int *ijk)
     return 0;
 }
 int cfd_port_samples(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_samples(port, host); }); }
+int cfd_port_gather(cfd_sim *s, int port, double *host)
+{
+    NEED(s);
+    if (port < 0 || port >= (int)s->impl->m.ports.size() || !host) { cfd::g_error = "no such port"; return -1; }
+    return guard([&] { s->impl->port_gather(port, host); });
+}
 int cfd_port_dft(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_dft(port, host); }); }
 int cfd_port_intensity(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_intensity(port, host); }); }
 int cfd_gain_read(cfd_sim *s, double *host) { NEED(s); return guard([&] { s->impl->gain_read(host); }); }

@@ -665,6 +665,17 @@ __global__ __launch_bounds__(256) void k_tfsf(Dev<R> d, const TfTarget *tg, int 
     F[g.cell] = v;
 }
 
+// the six field values at a list of cells, [point][6] (Ports.cpp:88-99 gather
+// for a caller that reads the current values)
+template <class R>
+__global__ __launch_bounds__(256) void k_gather(Dev<R> d, const long long *cell, int npts, R *out)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= npts) return;
+    const long long p = cell[t];
+    for (int c = 0; c < 6; c++) out[(long long)t * 6 + c] = d.F[c][p];
+}
+
 // soft point sources (main.cpp:284-288)
 template <class R>
 __global__ void k_sources(Dev<R> d, const long long *cell, const int *comp, const double *wave, int nsrc, int stride, int n)

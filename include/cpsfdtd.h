@@ -102,6 +102,9 @@ int cfd_read_id(cfd_sim *s, int16_t *host);                     /* Structure::ID
 int cfd_port_npts(cfd_sim *s, int port);
 int cfd_port_points(cfd_sim *s, int port, int *ijk);
 int cfd_port_samples(cfd_sim *s, int port, double *host);       /* [steps][npts][6] */
+int cfd_port_gather(cfd_sim *s, int port, double *host);        /* [npts][6] current values:
+                                                                   *Port::Update_Fields_on_Port Ports.cpp:88-99,342-353
+                                                                   read back (port_point[ii].Ex ... .Hz) */
 int cfd_port_dft(cfd_sim *s, int port, double *host);           /* [npts][n_freq][6][2] */
 int cfd_port_intensity(cfd_sim *s, int port, double *host);     /* [steps] sum of |E|^2 */
 int cfd_gain_read(cfd_sim *s, double *host);                    /* [cells][4] N0..N3 */

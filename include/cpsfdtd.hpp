@@ -1,0 +1,621 @@
+/*
+ * cpsfdtd.hpp -- C++ facade with the reference's class names and call
+ * signatures, over the C ABI of libcpsfdtd.so (cpsfdtd.h).
+ *
+ * A driver written against the reference's classes (main.cpp:115-343) keeps
+ * its shape:
+ *
+ *     cpsfdtd::Structure S("material.txt");          // Structure.h:35
+ *     S.generate_mesh("structure.txt");              // Structure.h:37
+ *     S.InitializeBasicFDTDCoefficients();           // Structure.h:38
+ *     cpsfdtd::Field F(S, "3D");                     // Fields.h:26
+ *     cpsfdtd::CPML pml(S, "CFS_PML.txt");           // CPML.h:25
+ *     pml.InitializeCPML(S);
+ *     cpsfdtd::TF_SF tfsf(S, "TF_SF.txt");           // TF_SF.h:32
+ *     cpsfdtd::PlainPort port(S, "port.txt");        // Ports.h:75
+ *     for (int n = 0; n < nt; ++n) {                 // main.cpp:267-343
+ *         cpsfdtd::BasicHFieldUpdate(F, S);
+ *         tfsf.UpdateHField(n, F, S);
+ *         pml.UpdateCPMLHFields(F);
+ *         cpsfdtd::BasicEFieldUpdate(F, S);
+ *         tfsf.UpdateEField(n, F, S);
+ *         pml.UpdateCPMLEFields(F);
+ *         port.Update_Fields_on_Port(F);
+ *     }
+ *
+ * Differences a caller sees:
+ *  - Objects register their input file; the device simulation is created by
+ *    the first update call, from every object constructed before it.  A
+ *    module constructed after that throws.
+ *  - Time steps are counted at H half-steps: every BasicHFieldUpdate after
+ *    the first one starts a new step (the reference's loop order).
+ *  - F.Ex[i][j][k] reads and writes a host mirror of the device field,
+ *    refreshed after device updates and written back before the next one
+ *    (the reference's public double*** arrays, Fields.h:29-34).
+ *  - Errors throw cpsfdtd::Error with the library's message (the reference
+ *    checks nothing).
+ *  - Ports have no trailing unset point (the reference's Num_of_Port_Points
+ *    counts one point it never sets, Ports.cpp:74,238,259,280).
+ *  - Options the reference hard-codes or lacks (precision, number of steps,
+ *    defect switches, periodic axes, point sources, port DFT, keeping every
+ *    port sample) are set on the Structure before the first update.
+ */
+#ifndef CPSFDTD_HPP
+#define CPSFDTD_HPP
+
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cpsfdtd.h"
+
+namespace cpsfdtd {
+
+class Error : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+class Field;
+
+namespace detail {
+
+inline void check(int rc, const char *what)
+{
+    if (rc < 0) throw Error(std::string(what) + ": " + cfd_last_error());
+}
+
+// One run: the files the objects registered, the options, and the device
+// simulation once it exists.
+struct Context {
+    std::string material, structure, pml, tfsf, dl, gain, source;
+    std::vector<std::string> port_files;
+    std::vector<int> port_kinds;
+    std::vector<int> src_ijkc;
+    std::vector<double> src_amp, src_phase, freqs;
+    int dft_stride = 1, record_ports = 0, nt = 0, periodic = 0, precision = CFD_F64;
+    unsigned flags = 0;
+    cfd_sim *sim = nullptr;
+    Field *field = nullptr;
+    long generation = 0;   // bumped by every call that may change device fields
+    int step = 0;          // steps begun so far minus one (index of the current step)
+    bool h_started = false;
+    int sampled_step = -1; // step whose port samples were taken
+
+    Context() = default;
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    ~Context()
+    {
+        if (sim) cfd_destroy(sim);
+    }
+    void require_unstarted(const char *what) const
+    {
+        if (sim) throw Error(std::string(what) + " registered after the simulation started");
+    }
+    cfd_sim *get()
+    {
+        if (sim) return sim;
+        std::vector<const char *> pf;
+        for (const std::string &p : port_files) pf.push_back(p.c_str());
+        cfd_config c{};
+        auto opt = [](const std::string &s) { return s.empty() ? nullptr : s.c_str(); };
+        c.material = material.c_str();
+        c.structure = structure.c_str();
+        c.pml = opt(pml);
+        c.tfsf = opt(tfsf);
+        c.dl = opt(dl);
+        c.gain = opt(gain);
+        c.source = opt(source);
+        c.n_ports = (int)pf.size();
+        c.port_files = pf.empty() ? nullptr : pf.data();
+        c.port_kinds = port_kinds.empty() ? nullptr : port_kinds.data();
+        c.n_src = (int)src_amp.size();
+        c.src_ijkc = src_ijkc.empty() ? nullptr : src_ijkc.data();
+        c.src_amp = src_amp.empty() ? nullptr : src_amp.data();
+        c.src_phase = src_phase.empty() ? nullptr : src_phase.data();
+        c.nt = nt;
+        c.n_freq = (int)freqs.size();
+        c.freqs = freqs.empty() ? nullptr : freqs.data();
+        c.dft_stride = dft_stride;
+        c.record_ports = record_ports;
+        c.periodic = periodic;
+        c.flags = flags;
+        c.precision = precision;
+        check(cfd_create(&c, &sim), "cfd_create");
+        return sim;
+    }
+    template <class Fn>
+    void device(Fn &&fn, const char *what);  // flush host writes, run, bump generation
+};
+
+}  // namespace detail
+
+// ---------------------------------------------------------------------------
+// Structure (Structure.h:33-60): media, mesh, dimensions, ID map
+// ---------------------------------------------------------------------------
+class Structure {
+public:
+    explicit Structure(const char *fileName) : ctx(std::make_shared<detail::Context>()) { ctx->material = fileName; }
+    Structure(const Structure &) = delete;
+    Structure &operator=(const Structure &) = delete;
+
+    void generate_mesh(const char *fileName)
+    {
+        ctx->require_unstarted("Structure::generate_mesh");
+        ctx->structure = fileName;
+        cfd_config c{};
+        c.material = ctx->material.c_str();
+        c.structure = ctx->structure.c_str();
+        c.nt = 1;
+        c.precision = CFD_F64;
+        cfd_model *m = nullptr;
+        if (cfd_model_build(&c, &m) < 0) throw Error(std::string("generate_mesh: ") + cfd_model_last_error());
+        int n[4];
+        double d[4];
+        cfd_model_dims(m, n, d);
+        Nx = n[0]; Ny = n[1]; Nz = n[2];
+        dx = d[0]; dy = d[1]; dz = d[2]; dt = d[3];
+        id_.assign((size_t)(Nx + 1) * (Ny + 1) * (Nz + 1), 0);
+        cfd_model_id(m, id_.data());
+        cfd_model_free(m);
+    }
+    void InitializeBasicFDTDCoefficients() {}  // the coefficients are built with the simulation
+
+    // ID[i][j][k] (Structure.h:43), read-only
+    struct IdRow2 {
+        const Structure *s; int i, j;
+        short operator[](int k) const { return s->id_[((size_t)i * (s->Ny + 1) + j) * (s->Nz + 1) + k]; }
+    };
+    struct IdRow1 {
+        const Structure *s; int i;
+        IdRow2 operator[](int j) const { return IdRow2{s, i, j}; }
+    };
+    struct IdMap {
+        const Structure *s;
+        IdRow1 operator[](int i) const { return IdRow1{s, i}; }
+    };
+    IdMap ID{this};
+
+    double dx = 0, dy = 0, dz = 0, dt = 0;
+    int Nx = 0, Ny = 0, Nz = 0;
+
+    // options the reference hard-codes or lacks; set before the first update
+    void set_precision(int bits) { ctx->require_unstarted("precision"); ctx->precision = bits; }
+    void set_steps(int nt) { ctx->require_unstarted("steps"); ctx->nt = nt; }
+    void set_flags(unsigned flags) { ctx->require_unstarted("flags"); ctx->flags = flags; }
+    void set_periodic(int axes) { ctx->require_unstarted("periodic axes"); ctx->periodic = axes; }
+    void set_source_file(const char *fileName) { ctx->require_unstarted("source"); ctx->source = fileName; }
+    void add_point_source(int i, int j, int k, int comp, double amp, double phase = 0.0)
+    {
+        ctx->require_unstarted("point source");
+        ctx->src_ijkc.insert(ctx->src_ijkc.end(), {i, j, k, comp});
+        ctx->src_amp.push_back(amp);
+        ctx->src_phase.push_back(phase);
+    }
+    void set_port_dft(const std::vector<double> &freqs_hz, int stride)
+    {
+        ctx->require_unstarted("port DFT");
+        ctx->freqs = freqs_hz;
+        ctx->dft_stride = stride;
+    }
+    void record_port_samples(bool on) { ctx->require_unstarted("port recording"); ctx->record_ports = on ? 1 : 0; }
+
+    cfd_sim *simulation() { return ctx->get(); }  // the device simulation, created on first use
+    int step() const { return ctx->step; }
+
+    std::shared_ptr<detail::Context> ctx;
+
+private:
+    std::vector<int16_t> id_;
+};
+
+// ---------------------------------------------------------------------------
+// Field (Fields.h:23-45): Ex..Hz with [i][j][k] access through host mirrors
+// ---------------------------------------------------------------------------
+class Field {
+public:
+    Field(Structure &sS, const char *smode = "3D") : ctx(sS.ctx), mode(smode ? smode : "")
+    {
+        if (ctx->field) throw Error("one Field per Structure");
+        ctx->field = this;
+        ncell = (size_t)(sS.Nx + 1) * (sS.Ny + 1) * (sS.Nz + 1);
+        ny1 = sS.Ny + 1;
+        nz1 = sS.Nz + 1;
+        if (!ncell || !sS.Nx) throw Error("Field: generate_mesh has not run");
+    }
+    ~Field()
+    {
+        if (ctx->field == this) ctx->field = nullptr;
+    }
+    Field(const Field &) = delete;
+    Field &operator=(const Field &) = delete;
+
+    class Ref {
+    public:
+        operator double() const { return f->host(c)[p]; }
+        Ref &operator=(double v)
+        {
+            f->host(c)[p] = v;
+            f->dirty[c] = true;
+            return *this;
+        }
+        Ref &operator+=(double v) { return *this = (double)*this + v; }
+        Ref(Field *f_, int c_, size_t p_) : f(f_), c(c_), p(p_) {}
+    private:
+        Field *f; int c; size_t p;
+    };
+    struct Row2 {
+        Field *f; int c; size_t base;
+        Ref operator[](int k) const { return Ref(f, c, base + k); }
+    };
+    struct Row1 {
+        Field *f; int c; int i;
+        Row2 operator[](int j) const { return Row2{f, c, ((size_t)i * f->ny1 + j) * f->nz1}; }
+    };
+    struct Component {
+        Field *f; int c;
+        Row1 operator[](int i) const { return Row1{f, c, i}; }
+    };
+    Component Ex{this, 0}, Ey{this, 1}, Ez{this, 2}, Hx{this, 3}, Hy{this, 4}, Hz{this, 5};
+
+    // whole component, [i][j][k] flat, k fastest
+    const std::vector<double> &values(int comp) { return host(comp); }
+
+    // write host changes back to the device (also done before every update)
+    void flush()
+    {
+        if (!ctx->sim) return;
+        for (int c = 0; c < 6; c++)
+            if (dirty[c]) {
+                detail::check(cfd_write_field(ctx->sim, c, mirror[c].data()), "cfd_write_field");
+                dirty[c] = false;
+            }
+    }
+
+private:
+    std::vector<double> &host(int c)
+    {
+        std::vector<double> &v = mirror[c];
+        if (v.size() != ncell) {
+            v.assign(ncell, 0.0);
+            valid[c] = -1;
+        }
+        if (ctx->sim && valid[c] != ctx->generation && !dirty[c]) {
+            detail::check(cfd_read_field(ctx->sim, c, v.data()), "cfd_read_field");
+            valid[c] = ctx->generation;
+        }
+        return v;
+    }
+
+    std::shared_ptr<detail::Context> ctx;
+    std::string mode;
+    size_t ncell = 0;
+    int ny1 = 0, nz1 = 0;
+    std::vector<double> mirror[6];
+    long valid[6] = {-1, -1, -1, -1, -1, -1};
+    bool dirty[6] = {false, false, false, false, false, false};
+};
+
+template <class Fn>
+void detail::Context::device(Fn &&fn, const char *what)
+{
+    cfd_sim *s = get();
+    if (field) field->flush();
+    check(fn(s), what);
+    generation++;
+}
+
+// ---------------------------------------------------------------------------
+// BasicFDTD.h:23-24
+// ---------------------------------------------------------------------------
+inline void BasicHFieldUpdate(Field &, Structure &sS)
+{
+    detail::Context &c = *sS.ctx;
+    if (c.h_started) {  // a new time step begins with its H half-step
+        c.device([](cfd_sim *s) { return cfd_advance(s); }, "cfd_advance");
+        c.step++;
+    }
+    c.h_started = true;
+    c.device([](cfd_sim *s) { return cfd_basic_h(s); }, "BasicHFieldUpdate");
+}
+inline void BasicEFieldUpdate(Field &, Structure &sS)
+{
+    sS.ctx->device([](cfd_sim *s) { return cfd_basic_e(s); }, "BasicEFieldUpdate");
+}
+
+// ---------------------------------------------------------------------------
+// CPML (CPML.h:22-30) and CFS_PML (CFS_PML.h:28-37): same physics
+// ---------------------------------------------------------------------------
+class CPML {
+public:
+    CPML(Structure &sS, const char *fileName) : ctx(sS.ctx)
+    {
+        ctx->require_unstarted("CPML");
+        ctx->pml = fileName;
+    }
+    void InitializeCPML(Structure &) {}
+    void UpdateCPMLEFields(Field &) { ctx->device([](cfd_sim *s) { return cfd_cpml_e(s); }, "UpdateCPMLEFields"); }
+    void UpdateCPMLHFields(Field &) { ctx->device([](cfd_sim *s) { return cfd_cpml_h(s); }, "UpdateCPMLHFields"); }
+private:
+    std::shared_ptr<detail::Context> ctx;
+};
+
+class CFS_PML {
+public:
+    CFS_PML(Structure &sS, const char *fileName) : ctx(sS.ctx)
+    {
+        ctx->require_unstarted("CFS_PML");
+        ctx->pml = fileName;
+    }
+    void Initialize_PML(Structure &) {}
+    void UpdatePMLEFields(Field &) { ctx->device([](cfd_sim *s) { return cfd_cpml_e(s); }, "UpdatePMLEFields"); }
+    void UpdatePMLHFields(Field &) { ctx->device([](cfd_sim *s) { return cfd_cpml_h(s); }, "UpdatePMLHFields"); }
+private:
+    std::shared_ptr<detail::Context> ctx;
+};
+
+// ---------------------------------------------------------------------------
+// TF_SF (TF_SF.h:30-58): face corrections at time step n
+// ---------------------------------------------------------------------------
+class TF_SF {
+public:
+    TF_SF(Structure &sS, const char *fileName) : ctx(sS.ctx)
+    {
+        ctx->require_unstarted("TF_SF");
+        ctx->tfsf = fileName;
+    }
+    void UpdateEField(int timestep, Field &, Structure &)
+    {
+        ctx->device([timestep](cfd_sim *s) { return cfd_tfsf_e(s, timestep); }, "TF_SF::UpdateEField");
+    }
+    void UpdateHField(int timestep, Field &, Structure &)
+    {
+        ctx->device([timestep](cfd_sim *s) { return cfd_tfsf_h(s, timestep); }, "TF_SF::UpdateHField");
+    }
+private:
+    std::shared_ptr<detail::Context> ctx;
+};
+
+// ---------------------------------------------------------------------------
+// Drude_Lorentz (Drude_Lorentz.h:28-45), 3-D, on ID 2 cells
+// ---------------------------------------------------------------------------
+class Drude_Lorentz {
+public:
+    Drude_Lorentz(Structure &sS, const char *fileName) : ctx(sS.ctx)
+    {
+        ctx->require_unstarted("Drude_Lorentz");
+        ctx->dl = fileName;
+    }
+    void EFieldUpdate(Field &, Structure &) { ctx->device([](cfd_sim *s) { return cfd_dl_e(s); }, "Drude_Lorentz::EFieldUpdate"); }
+private:
+    std::shared_ptr<detail::Context> ctx;
+};
+
+// ---------------------------------------------------------------------------
+// FourLevelLaser (FourLevelLaser.h:20-33), 3-D, on ID 5 cells
+// ---------------------------------------------------------------------------
+class FourLevelLaser {
+public:
+    FourLevelLaser(Structure &sS, const char *fileName) : ctx(sS.ctx)
+    {
+        ctx->require_unstarted("FourLevelLaser");
+        ctx->gain = fileName;
+    }
+    void UpdatePolarizationDensities(Field &) { ctx->device([](cfd_sim *s) { return cfd_gain_pol(s); }, "UpdatePolarizationDensities"); }
+    void UpdateElectricField(Field &) { ctx->device([](cfd_sim *s) { return cfd_gain_e(s); }, "UpdateElectricField"); }
+    void UpdateStatesPopulations(Field &) { ctx->device([](cfd_sim *s) { return cfd_gain_pop(s); }, "UpdateStatesPopulations"); }
+    // populations of every gain cell, [cell][N0..N3]
+    std::vector<double> populations()
+    {
+        int counts[4];
+        detail::check(cfd_info(ctx->get(), nullptr, nullptr, counts), "cfd_info");
+        std::vector<double> v((size_t)counts[2] * 4);
+        detail::check(cfd_gain_read(ctx->sim, v.data()), "cfd_gain_read");
+        return v;
+    }
+private:
+    std::shared_ptr<detail::Context> ctx;
+};
+
+// soft point sources of the Structure's options at step n (main.cpp:284-288)
+inline void UpdateSources(Structure &sS, int timestep)
+{
+    sS.ctx->device([timestep](cfd_sim *s) { return cfd_sources(s, timestep); }, "sources");
+}
+
+// ---------------------------------------------------------------------------
+// Ports (Ports.h:20-92)
+// ---------------------------------------------------------------------------
+struct Port_Points {
+    int i, j, k;
+    double Ex, Ey, Ez, Hx, Hy, Hz;
+};
+
+namespace detail {
+
+class PortBase {
+public:
+    std::vector<Port_Points> port_point;
+    int Num_of_Port_Points = 0;
+
+    // Ports.cpp:88-99 / 342-353: this step's six values at every point (the
+    // device also records them, and feeds the running DFT when configured)
+    void Update_Fields_on_Port(Field &)
+    {
+        Context &c = *ctx;
+        cfd_sim *s = c.get();
+        if (c.field) c.field->flush();
+        if (c.sampled_step != c.step) {
+            check(cfd_ports_sample(s, c.step), "cfd_ports_sample");
+            c.sampled_step = c.step;
+        }
+        points();
+        std::vector<double> v(port_point.size() * 6);
+        if (!v.empty()) check(cfd_port_gather(s, index, v.data()), "cfd_port_gather");
+        for (size_t t = 0; t < port_point.size(); t++) {
+            Port_Points &q = port_point[t];
+            q.Ex = v[t * 6]; q.Ey = v[t * 6 + 1]; q.Ez = v[t * 6 + 2];
+            q.Hx = v[t * 6 + 3]; q.Hy = v[t * 6 + 4]; q.Hz = v[t * 6 + 5];
+        }
+    }
+    // Ports.cpp:112-119: one line per point, Ex Ey Ez
+    void Export_E_onPort(const char *fileName)
+    {
+        std::ofstream out(fileName);
+        Export_E_onPort(out);
+    }
+    void Export_E_onPort(std::ofstream &out)
+    {
+        for (const Port_Points &q : port_point) out << q.Ex << "\t" << q.Ey << "\t" << q.Ez << std::endl;
+    }
+    // the port's point list, i j k per line (Ports.cpp:356-362)
+    void points_to(std::ostream &out)
+    {
+        points();
+        for (const Port_Points &q : port_point) out << q.i << "\t" << q.j << "\t" << q.k << std::endl;
+    }
+    int port_index() const { return index; }
+
+protected:
+    PortBase(Structure &sS, const char *fileName, int kind) : ctx(sS.ctx)
+    {
+        ctx->require_unstarted("port");
+        index = (int)ctx->port_files.size();
+        ctx->port_files.push_back(fileName);
+        ctx->port_kinds.push_back(kind);
+    }
+    void points()
+    {
+        if (!port_point.empty() || !ctx->get()) return;
+        const int n = cfd_port_npts(ctx->sim, index);
+        check(n, "cfd_port_npts");
+        std::vector<int> ijk((size_t)n * 3);
+        if (n) check(cfd_port_points(ctx->sim, index, ijk.data()), "cfd_port_points");
+        port_point.resize(n);
+        for (int t = 0; t < n; t++) port_point[t] = Port_Points{ijk[3 * t], ijk[3 * t + 1], ijk[3 * t + 2], 0, 0, 0, 0, 0, 0};
+        Num_of_Port_Points = n;
+    }
+    template <class Get>
+    void row(std::ostream &out, Get get, int nw, bool per_row_newline)
+    {
+        // Ports.cpp:377-388 (plane ports: rows of Nw values) and 121-143
+        // (sphere ports: one row); values separated and ended by a tab
+        int ii = 0;
+        const int n = (int)port_point.size();
+        while (ii < n) {
+            for (int i = 0; i < nw && ii < n; ++i, ++ii) out << get(port_point[ii]) << "\t";
+            if (per_row_newline) out << std::endl;
+        }
+    }
+    std::shared_ptr<Context> ctx;
+    int index = 0;
+};
+
+}  // namespace detail
+
+class SphericalPort : public detail::PortBase {
+public:
+    SphericalPort(Structure &sS, const char *fileName) : PortBase(sS, fileName, CFD_PORT_SPHERE), dx(sS.dx), dy(sS.dy), dz(sS.dz)
+    {
+        std::ifstream in(fileName);
+        std::string line;
+        std::getline(in, line);  // r  (Ports.cpp:40-43)
+        std::istringstream(line) >> r;
+        std::getline(in, line);  // xc yc zc
+        std::istringstream(line) >> xc >> yc >> zc;
+    }
+    void Export_Port(const char *fileName)  // Ports.cpp:101-109: centre in cells, then the points
+    {
+        std::ofstream out(fileName);
+        out << xc / dx << "\t" << yc / dy << "\t" << zc / dz << std::endl;
+        points_to(out);
+    }
+    void Export_Ex_onPort(const char *f) { file_row(f, [](const Port_Points &q) { return q.Ex; }); }
+    void Export_Ey_onPort(const char *f) { file_row(f, [](const Port_Points &q) { return q.Ey; }); }
+    void Export_Ez_onPort(const char *f) { file_row(f, [](const Port_Points &q) { return q.Ez; }); }
+    void Export_EIntensity_onPort(const char *f) { file_row(f, intensity); }
+    void Export_Ex_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ex; }); }
+    void Export_Ey_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ey; }); }
+    void Export_Ez_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ez; }); }
+    void Export_EIntensity_onPort(std::ofstream &o) { step_row(o, intensity); }
+    using PortBase::Export_E_onPort;
+
+    double r = 0, xc = 0, yc = 0, zc = 0;
+
+private:
+    static double intensity(const Port_Points &q) { return q.Ex * q.Ex + q.Ey * q.Ey + q.Ez * q.Ez; }
+    template <class Get>
+    void file_row(const char *f, Get g)  // Ports.cpp:130-137: values, no newline
+    {
+        std::ofstream out(f);
+        row(out, g, (int)port_point.size(), false);
+    }
+    template <class Get>
+    void step_row(std::ofstream &o, Get g)  // Ports.cpp:138-144: one line per call
+    {
+        row(o, g, (int)port_point.size(), false);
+        o << std::endl;
+    }
+    double dx, dy, dz;
+};
+
+class PlainPort : public detail::PortBase {
+public:
+    PlainPort(Structure &sS, const char *fileName) : PortBase(sS, fileName, CFD_PORT_PLANE) {}
+    void Export_Port(const char *fileName)  // Ports.cpp:355-362
+    {
+        std::ofstream out(fileName);
+        points_to(out);
+    }
+    void Export_Ex_onPort(const char *f) { file_rows(f, [](const Port_Points &q) { return q.Ex; }); }
+    void Export_Ey_onPort(const char *f) { file_rows(f, [](const Port_Points &q) { return q.Ey; }); }
+    void Export_Ez_onPort(const char *f) { file_rows(f, [](const Port_Points &q) { return q.Ez; }); }
+    void Export_EIntensity_onPort(const char *f) { file_rows(f, intensity); }
+    void Export_Ex_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ex; }); }
+    void Export_Ey_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ey; }); }
+    void Export_Ez_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ez; }); }
+    void Export_EIntensity_onPort(std::ofstream &o) { step_row(o, intensity); }
+    using PortBase::Export_E_onPort;
+
+    // points along the port's first in-plane axis (Ports.cpp:232-295 Nw)
+    int Nw()
+    {
+        points();
+        const int n = (int)port_point.size();
+        if (n < 2) return n;
+        const Port_Points &a = port_point[0], &b = port_point[1];
+        const int ax = a.i != b.i ? 0 : a.j != b.j ? 1 : 2;
+        int w = 1;
+        while (w < n && coord(port_point[w], ax) != coord(a, ax) && other_equal(port_point[w], a, ax)) w++;
+        return w;
+    }
+
+private:
+    static double intensity(const Port_Points &q) { return q.Ex * q.Ex + q.Ey * q.Ey + q.Ez * q.Ez; }
+    static int coord(const Port_Points &q, int ax) { return ax == 0 ? q.i : ax == 1 ? q.j : q.k; }
+    static bool other_equal(const Port_Points &q, const Port_Points &a, int ax)
+    {
+        return (ax == 0 || q.i == a.i) && (ax == 1 || q.j == a.j) && (ax == 2 || q.k == a.k);
+    }
+    template <class Get>
+    void file_rows(const char *f, Get g)  // Ports.cpp:377-388: Nw values per line
+    {
+        std::ofstream out(f);
+        row(out, g, Nw(), true);
+    }
+    template <class Get>
+    void step_row(std::ofstream &o, Get g)  // Ports.cpp:389-400: the whole port on one line
+    {
+        row(o, g, (int)port_point.size(), false);
+        o << std::endl;
+    }
+};
+
+}  // namespace cpsfdtd
+
+#endif  // CPSFDTD_HPP

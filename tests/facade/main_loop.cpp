@@ -1,0 +1,148 @@
+// main_loop.cpp -- a driver in the shape of the reference's main.cpp time
+// loop (main.cpp:115-343), written against the C++ facade (cpsfdtd.hpp).
+// The tests run it and compare what it writes with the FP64 oracle.
+//
+//   main_loop --material F --structure F [--pml F] [--tfsf F] [--dl F]
+//             [--gain F] [--source F] [--plane F]... [--sphere F]...
+//             [--src i j k comp amp] [--steps N] [--flags U] [--out DIR]
+//             [--mesh-only]
+//
+// --mesh-only stops after Structure::generate_mesh (no device): writes
+// DIR/id.i16 (ID map, [i][j][k]) and prints Nx Ny Nz dx dy dz dt.
+// Writes DIR/field_{Ex..Hz}.f64 (final fields, [i][j][k]), per port q:
+// DIR/port{q}_points.txt (Export_Port), DIR/port{q}_Ex.txt (one line per
+// step, Export_Ex_onPort(ofstream&)), DIR/port{q}_E_last.txt
+// (Export_E_onPort(file) after the last step), DIR/port{q}_samples.f64
+// ([step][point][6] from port_point after Update_Fields_on_Port).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cpsfdtd.hpp"
+
+int main(int argc, char **argv)
+{
+    std::string material, structure, pml, tfsf, dl, gain, source, out = ".";
+    std::vector<std::string> planes, spheres;
+    std::vector<int> src_ijkc;
+    std::vector<double> src_amp;
+    int steps = 10;
+    unsigned flags = 0;
+    bool mesh_only = false;
+    for (int a = 1; a < argc; a++) {
+        std::string k = argv[a];
+        auto next = [&]() -> std::string {
+            if (a + 1 >= argc) { std::fprintf(stderr, "missing value for %s\n", k.c_str()); std::exit(2); }
+            return argv[++a];
+        };
+        if (k == "--material") material = next();
+        else if (k == "--structure") structure = next();
+        else if (k == "--pml") pml = next();
+        else if (k == "--tfsf") tfsf = next();
+        else if (k == "--dl") dl = next();
+        else if (k == "--gain") gain = next();
+        else if (k == "--source") source = next();
+        else if (k == "--plane") planes.push_back(next());
+        else if (k == "--sphere") spheres.push_back(next());
+        else if (k == "--steps") steps = std::atoi(next().c_str());
+        else if (k == "--flags") flags = (unsigned)std::strtoul(next().c_str(), nullptr, 0);
+        else if (k == "--out") out = next();
+        else if (k == "--mesh-only") mesh_only = true;
+        else if (k == "--src") {
+            for (int q = 0; q < 4; q++) src_ijkc.push_back(std::atoi(next().c_str()));
+            src_amp.push_back(std::atof(next().c_str()));
+        } else { std::fprintf(stderr, "unknown option %s\n", k.c_str()); return 2; }
+    }
+    try {
+        cpsfdtd::Structure S(material.c_str());
+        S.generate_mesh(structure.c_str());
+        S.InitializeBasicFDTDCoefficients();
+        if (mesh_only) {
+            std::vector<int16_t> id;
+            for (int i = 0; i <= S.Nx; i++)
+                for (int j = 0; j <= S.Ny; j++)
+                    for (int k = 0; k <= S.Nz; k++) id.push_back(S.ID[i][j][k]);
+            FILE *fh = std::fopen((out + "/id.i16").c_str(), "wb");
+            std::fwrite(id.data(), sizeof(int16_t), id.size(), fh);
+            std::fclose(fh);
+            std::printf("%d %d %d %.17g %.17g %.17g %.17g\n", S.Nx, S.Ny, S.Nz, S.dx, S.dy, S.dz, S.dt);
+            return 0;
+        }
+        S.set_steps(steps);
+        S.set_flags(flags);
+        if (!source.empty()) S.set_source_file(source.c_str());
+        for (size_t q = 0; q < src_amp.size(); q++)
+            S.add_point_source(src_ijkc[4 * q], src_ijkc[4 * q + 1], src_ijkc[4 * q + 2], src_ijkc[4 * q + 3], src_amp[q]);
+        cpsfdtd::Field F(S, "3D");
+        std::unique_ptr<cpsfdtd::CPML> cpml;
+        if (!pml.empty()) { cpml.reset(new cpsfdtd::CPML(S, pml.c_str())); cpml->InitializeCPML(S); }
+        std::unique_ptr<cpsfdtd::TF_SF> tf;
+        if (!tfsf.empty()) tf.reset(new cpsfdtd::TF_SF(S, tfsf.c_str()));
+        std::unique_ptr<cpsfdtd::Drude_Lorentz> DL;
+        if (!dl.empty()) DL.reset(new cpsfdtd::Drude_Lorentz(S, dl.c_str()));
+        std::unique_ptr<cpsfdtd::FourLevelLaser> laser;
+        if (!gain.empty()) laser.reset(new cpsfdtd::FourLevelLaser(S, gain.c_str()));
+        std::vector<std::unique_ptr<cpsfdtd::PlainPort>> pp;
+        std::vector<std::unique_ptr<cpsfdtd::SphericalPort>> sp;
+        for (const std::string &f : planes) pp.emplace_back(new cpsfdtd::PlainPort(S, f.c_str()));
+        for (const std::string &f : spheres) sp.emplace_back(new cpsfdtd::SphericalPort(S, f.c_str()));
+
+        const size_t nports = pp.size() + sp.size();
+        std::vector<std::unique_ptr<std::ofstream>> exOut;
+        std::vector<std::vector<double>> samples(nports);
+        for (size_t q = 0; q < nports; q++)
+            exOut.emplace_back(new std::ofstream(out + "/port" + std::to_string(q) + "_Ex.txt"));
+        for (size_t q = 0; q < pp.size(); q++) pp[q]->Export_Port((out + "/port" + std::to_string(q) + "_points.txt").c_str());
+        for (size_t q = 0; q < sp.size(); q++)
+            sp[q]->Export_Port((out + "/port" + std::to_string(pp.size() + q) + "_points.txt").c_str());
+
+        for (int n = 0; n < steps; ++n) {
+            cpsfdtd::BasicHFieldUpdate(F, S);
+            if (tf) tf->UpdateHField(n, F, S);
+            if (cpml) cpml->UpdateCPMLHFields(F);
+            if (laser) laser->UpdatePolarizationDensities(F);
+            cpsfdtd::BasicEFieldUpdate(F, S);
+            if (DL) DL->EFieldUpdate(F, S);
+            if (laser) { laser->UpdateElectricField(F); laser->UpdateStatesPopulations(F); }
+            if (tf) tf->UpdateEField(n, F, S);
+            if (!src_amp.empty()) cpsfdtd::UpdateSources(S, n);
+            if (cpml) cpml->UpdateCPMLEFields(F);
+            for (size_t q = 0; q < nports; q++) {
+                cpsfdtd::detail::PortBase &P = q < pp.size() ? (cpsfdtd::detail::PortBase &)*pp[q]
+                                                              : (cpsfdtd::detail::PortBase &)*sp[q - pp.size()];
+                P.Update_Fields_on_Port(F);
+                if (q < pp.size()) pp[q]->Export_Ex_onPort(*exOut[q]);
+                else sp[q - pp.size()]->Export_Ex_onPort(*exOut[q]);
+                for (const cpsfdtd::Port_Points &t : P.port_point)
+                    samples[q].insert(samples[q].end(), {t.Ex, t.Ey, t.Ez, t.Hx, t.Hy, t.Hz});
+            }
+        }
+        for (size_t q = 0; q < nports; q++) {
+            exOut[q]->close();
+            cpsfdtd::detail::PortBase &P = q < pp.size() ? (cpsfdtd::detail::PortBase &)*pp[q]
+                                                          : (cpsfdtd::detail::PortBase &)*sp[q - pp.size()];
+            P.Export_E_onPort((out + "/port" + std::to_string(q) + "_E_last.txt").c_str());
+            FILE *fh = std::fopen((out + "/port" + std::to_string(q) + "_samples.f64").c_str(), "wb");
+            std::fwrite(samples[q].data(), sizeof(double), samples[q].size(), fh);
+            std::fclose(fh);
+        }
+        static const char *names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+        for (int c = 0; c < 6; c++) {
+            const std::vector<double> &v = F.values(c);
+            FILE *fh = std::fopen((out + "/field_" + names[c] + ".f64").c_str(), "wb");
+            std::fwrite(v.data(), sizeof(double), v.size(), fh);
+            std::fclose(fh);
+        }
+        // a read through the reference's array syntax, to exercise the proxies
+        std::printf("steps %d Ez[%d][%d][%d] = %.17g\n", S.step() + 1, S.Nx / 2, S.Ny / 2, S.Nz / 2,
+                    (double)F.Ez[S.Nx / 2][S.Ny / 2][S.Nz / 2]);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "main_loop: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,115 @@
+"""The C++ facade (include/cpsfdtd.hpp): the reference's class names and call
+sequence (main.cpp:115-343) over the C ABI.
+
+tests/facade/main_loop.cpp is a driver in the shape of the reference's main:
+Structure, Field, CPML, TF_SF, Drude_Lorentz, FourLevelLaser, PlainPort and
+SphericalPort objects, then the per-step module calls in the reference's
+order.  Its outputs (final fields, port values read through port_point, the
+text written by Export_Ex_onPort / Export_E_onPort / Export_Port) are checked
+against the FP64 oracle: bit for bit for values, character for character for
+text (default stream formatting of the same doubles).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+from cpsfdtd_amd import build as engine_build
+from cpsfdtd_amd.case import FIXED
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = engine_build.FACADE_DRIVER
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(DRIVER):
+        if not os.path.exists(engine_build.LIB):
+            pytest.skip("libcpsfdtd.so is not built")
+        engine_build.build_facade_driver()
+    return DRIVER
+
+
+def driver_args(case, steps, out):
+    f = case.files
+    args = [DRIVER, "--material", f["material"], "--structure", f["structure"], "--steps", str(steps),
+            "--flags", str(case.flags), "--out", str(out)]
+    for key in ("pml", "tfsf", "dl", "gain", "source"):
+        if f[key]:
+            args += ["--" + key, f[key]]
+    planes = [p for p, k in case.ports if k == 0]
+    spheres = [p for p, k in case.ports if k == 1]
+    assert [p for p, _ in case.ports] == planes + spheres, "driver orders plane ports first"
+    for p in planes:
+        args += ["--plane", p]
+    for p in spheres:
+        args += ["--sphere", p]
+    for s in case.sources:
+        assert s[5] == 0.0
+        args += ["--src"] + [str(v) for v in s[:4]] + [repr(float(s[4]))]
+    return args
+
+
+def test_header_compiles_standalone(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "cpsfdtd.hpp"\nint main() { return 0; }\n')
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                    "-I" + os.path.join(ROOT, "include"), str(src)], check=True)
+
+
+def test_structure_mesh_matches_oracle(tmp_path, driver, oracle_lib):
+    """Structure(material) + generate_mesh(structure) through the facade, no
+    device: dimensions, time step and the ID map equal the oracle's."""
+    case = cases.mixed(str(tmp_path / "case"))
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run(driver_args(case, 1, out) + ["--mesh-only"], check=True, capture_output=True, text=True)
+    n = r.stdout.split()
+    o = oracle_lib.OracleSim(case)
+    assert [int(v) for v in n[:3]] == list(o.N)
+    assert float(n[6]) == o.dt
+    want = o.id_map()
+    got = np.fromfile(out / "id.i16", dtype=np.int16).reshape(want.shape)
+    assert np.array_equal(got, want)
+
+
+def g6(v):
+    """std::ostream << double with the default format (%g, precision 6)."""
+    return "%g" % v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["ref", "dl_gain"])
+def test_main_loop_matches_oracle(tmp_path, driver, oracle_lib, variant):
+    if variant == "ref":
+        case = cases.mixed(str(tmp_path / "case"))
+    else:
+        case = cases.mixed(str(tmp_path / "case"), dl=True, gain=True, flags=FIXED)
+    steps = 40
+    out = tmp_path / "out"
+    out.mkdir()
+    print("facade main_loop", variant, flush=True)
+    subprocess.run(driver_args(case, steps, out), check=True, timeout=120)
+    o = oracle_lib.OracleSim(case)
+    o.step(steps)
+    for c, nm in enumerate(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")):
+        want = o.field(c)
+        got = np.fromfile(out / ("field_%s.f64" % nm), dtype=np.float64).reshape(want.shape)
+        assert np.array_equal(got, want), nm
+    for q in range(len(case.ports)):
+        want = o.port_samples(q)
+        got = np.fromfile(out / ("port%d_samples.f64" % q), dtype=np.float64).reshape(want.shape)
+        assert np.array_equal(got, want), q
+        rows = (out / ("port%d_Ex.txt" % q)).read_text().split("\n")
+        assert rows[-1] == "" and len(rows) == steps + 1
+        for n in (0, steps // 2, steps - 1):
+            assert rows[n] == "".join(g6(v) + "\t" for v in want[n, :, 0]), (q, n)
+        last = (out / ("port%d_E_last.txt" % q)).read_text()
+        assert last == "".join("%s\t%s\t%s\n" % (g6(a), g6(b), g6(c)) for a, b, c in want[-1, :, :3])
+        pts = o.port_points(q)
+        lines = (out / ("port%d_points.txt" % q)).read_text().splitlines()
+        if case.ports[q][1] == 1:
+            lines = lines[1:]  # sphere ports begin with the centre in cells
+        assert lines == ["%d\t%d\t%d" % tuple(p) for p in pts]
