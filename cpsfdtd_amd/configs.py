@@ -4,8 +4,8 @@ formats (so the reference's own parsers and the engine read the same files).
   c1  plain FP64 Yee, 128^3, dielectric sphere, Ez point source, line port
   c2  256^3, 4x4 Drude-Lorentz spheres, 12-cell CPML, oblique Gaussian-beam
       TF/SF, two plane ports with a 64-frequency running DFT
-  c3  192x192x64 disordered rod slab with four-level gain, periodic x/y
-  c4  batched periodic unit cells (see cpsfdtd_amd.batch)
+  c3  192x192x64 disordered rod slab with four-level gain, periodic x/y,
+      random-phase dipole, sphere port
 
 ``scale`` < 1 shrinks a workload isotropically for tests (cell counts, PML and
 box insets scale; physics parameters stay).
@@ -89,3 +89,90 @@ def c2_reference_core(directory, scale=1.0, nt=4000):
     case.ports = []
     case.freqs = []
     return case
+
+
+class MT19937_64:
+    """64-bit Mersenne Twister (Matsumoto & Nishimura, ACM TOMACS 10(4) 2000,
+    64-bit parameters of Nishimura 2000), as std::mt19937_64; seeds like
+    std::mt19937_64(seed).  uniform() = top 53 bits / 2^53."""
+    N, M = 312, 156
+    A = 0xB5026F5AA96619E9
+    UPPER, LOWER = 0xFFFFFFFF80000000, 0x7FFFFFFF
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed=5489):
+        mt = [seed & self.MASK]
+        for i in range(1, self.N):
+            mt.append((6364136223846793005 * (mt[-1] ^ (mt[-1] >> 62)) + i) & self.MASK)
+        self.mt, self.i = mt, self.N
+
+    def next(self):
+        if self.i >= self.N:
+            mt, n, m = self.mt, self.N, self.M
+            for k in range(n):
+                y = (mt[k] & self.UPPER) | (mt[(k + 1) % n] & self.LOWER)
+                mt[k] = mt[(k + m) % n] ^ (y >> 1) ^ (self.A if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & self.MASK
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def rod_positions(n, nx, ny, r, spacing, seed):
+    """Random sequential placement of n rod centres (cell units) in
+    [r+1, nx-r-1) x [r+1, ny-r-1): draw x then y from MT19937_64(seed), reject a
+    centre closer than ``spacing`` to an accepted one."""
+    rng = MT19937_64(seed)
+    pts = []
+    tries = 0
+    while len(pts) < n:
+        tries += 1
+        if tries > 1000 * n:
+            raise ValueError("cannot place %d rods with spacing %g" % (n, spacing))
+        x = (r + 1) + rng.uniform() * (nx - 2 * r - 2)
+        y = (r + 1) + rng.uniform() * (ny - 2 * r - 2)
+        if all((x - a) ** 2 + (y - b) ** 2 >= spacing * spacing for a, b in pts):
+            pts.append((x, y))
+    return pts
+
+
+def c3(directory, scale=1.0, nt=20000, flags=FIXED, seed=1234, record=False):
+    """192x192x64, dx=20 nm; a slab of four-level gain medium (host eps_r 2.25,
+    N_tot 3.3e24 m^-3, pump 1e9 /s, tau32 1e-13 s, tau21 1e-10 s, tau10 1e-13 s,
+    600 nm line, linewidth 1e14 rad/s) 24 cells thick in the middle of z,
+    pierced by 400 rods (eps_r 11.56, radius 3 cells, through the slab) placed
+    by rod_positions with seed 1234 and a 7-cell minimum spacing (8 cells, the
+    first choice, is at the random-packing limit for 400 rods in 192^2 cells);
+    x and y periodic, 10-cell CPML in z; one Ez dipole at the slab centre with
+    a seeded random phase (600 nm, 20 fs, 60 fs delay); a sphere port of
+    radius 60 cells over theta 70-110 deg around the centre."""
+    nx = ny = _s(192, scale, 24)
+    nz = _s(64, scale, 24)
+    d = (20e-9, 20e-9, 20e-9)
+    npml = _s(10, scale, 4)
+    h = _s(24, scale, 4)
+    k0 = (nz - h) // 2
+    r = _s(3, scale, 1)
+    nrods = _s(400, scale * scale, 4)
+    spacing = 7.0 * scale if scale < 1 else 7.0
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (11.56, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0), (2.25, 1, 0, 0)])
+    shapes = [Shape(5, "cuboid", x0=0.0, y0=0.0, z0=k0 * d[2], a=(nx + 1) * d[0], b=(ny + 1) * d[1], c=h * d[2])]
+    for x, y in rod_positions(nrods, nx, ny, r, spacing, seed):
+        shapes.append(Shape(1, "cylinder", x0=x * d[0], y0=y * d[1], z0=k0 * d[2], a=r * d[0], b=r * d[1], c=h * d[2]))
+    w.grid((nx, ny, nz), d, shapes)
+    w.cpml(((0, 0), (0, 0), (npml, npml)), m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.gain(600e-9, 1e14, 1e-13, 1e-10, 1e-13, 3.3e24, 1e9)
+    w.source(600e-9, 20e-15, 60e-15)
+    phase = 2 * math.pi * MT19937_64(seed + 1).uniform()
+    w.point_source(nx // 2, ny // 2, nz // 2, comp=2, amp=1.0, phase=phase)
+    rs = _s(60, scale, 4)
+    w.sphere_port("sphere.txt", rs * d[0], (nx // 2) * d[0], (ny // 2) * d[1], (nz // 2) * d[2], 70.0, 110.0, 0.0, 360.0)
+    return w.run(nt, record_ports=record, periodic=3, flags=flags)
