@@ -10,7 +10,10 @@ form runs beside it.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line.  Per-kernel durations come from HIP events recorded on
-the engine's stream around each launch (cfd_kernel_times).
+the engine's stream around each launch (cfd_kernel_times).  With the default
+--warmup 20 --steps 3940 --timing-steps 40 each precision runs the whole
+4000-step workload, and the FP32 results are compared with the FP64 ones
+(relative L2 of the port |E|^2 series and DFTs, spectral peak and Q).
 """
 from __future__ import annotations
 
@@ -110,8 +113,50 @@ def run_engine(case, precision, steps, warmup, timing_steps):
     kt = e.kernel_times()
     e.set_timing(False)
     nbytes = e.device_bytes()
+    results = [(e.port_intensity(q), e.port_dft(q)) for q in range(len(case.ports))]
     e.close()
-    return wall, kt, nbytes
+    return wall, kt, nbytes, results
+
+
+def spectrum_peak(freqs, dft):
+    """Peak frequency and Q = f0 / FWHM of a port's power spectrum
+    sum over points and E components of |E(f)|^2 (half-maximum crossings by
+    linear interpolation; None where the peak is not bracketed)."""
+    import numpy as np
+    f = np.asarray(freqs)
+    p = (np.abs(dft[:, :, :3]) ** 2).sum(axis=(0, 2))  # dft: complex [point][freq][6]
+    order = np.argsort(f)
+    f, p = f[order], p[order]
+    k = int(np.argmax(p))
+    half = p[k] / 2
+
+    def cross(rng):
+        for a in rng:
+            b = a + 1 if rng.step > 0 else a - 1
+            if 0 <= b < len(p) and (p[a] - half) * (p[b] - half) <= 0 and p[a] != p[b]:
+                return f[a] + (half - p[a]) * (f[b] - f[a]) / (p[b] - p[a])
+        return None
+    lo, hi = cross(range(k, 0, -1)), cross(range(k, len(p) - 1))
+    q = float(f[k] / (hi - lo)) if lo is not None and hi is not None and hi > lo else None
+    return float(f[k]), q
+
+
+def quality(res32, res64, freqs):
+    """FP32 against FP64 over the same run: relative L2 of every port's |E|^2
+    time series and DFT, and the transmitted port's spectral peak and Q."""
+    import numpy as np
+
+    def rel(a, b):
+        den = np.linalg.norm(np.ravel(b))
+        return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / den) if den > 0 else 0.0
+    out = {"port_series_rel_l2": [round(rel(a[0], b[0]), 9) for a, b in zip(res32, res64)],
+           "port_dft_rel_l2": [round(rel(a[1], b[1]), 9) for a, b in zip(res32, res64)]}
+    if freqs and res64:
+        f32, q32 = spectrum_peak(freqs, res32[-1][1])
+        f64, q64 = spectrum_peak(freqs, res64[-1][1])
+        out.update({"peak_thz_f32": round(f32 / 1e12, 4), "peak_thz_f64": round(f64 / 1e12, 4),
+                    "q_f32": q32 and round(q32, 4), "q_f64": q64 and round(q64, 4)})
+    return out
 
 
 def cpu_baseline(scale_steps=200):
@@ -144,7 +189,7 @@ def cpu_baseline(scale_steps=200):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=3940)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--no-fp64", action="store_true")
@@ -196,7 +241,7 @@ def main():
     for prec in ([32] if args.no_fp64 else [32, 64]):
         S = prec // 8
         log("bench: FP%d run, %d warmup + %d timed steps" % (prec, args.warmup, args.steps))
-        wall, kt, nbytes = run_engine(case, prec, args.steps, args.warmup, args.timing_steps)
+        wall, kt, nbytes, results = run_engine(case, prec, args.steps, args.warmup, args.timing_steps)
         ab = algorithmic_bytes(geo, S, nd_vars)
         kstat = {}
         for name, (ms, cnt) in kt.items():
@@ -207,9 +252,10 @@ def main():
         res[prec] = {
             "wall": wall, "mcells": cells * args.steps / wall / 1e6, "ms_per_step": wall / args.steps * 1e3,
             "kernels": kstat, "dominant": dom, "dom_gbs": ab[dom] / avg_s / 1e9, "dom_bytes": ab[dom],
-            "step_gbs": sum(ab.values()) / (wall / args.steps) / 1e9, "device_bytes": nbytes,
+            "step_gbs": sum(v for k, v in ab.items() if k in kt) / (wall / args.steps) / 1e9, "device_bytes": nbytes,
             "kernel_ms_per_step": step_ms,
             "pass_gbs": {k: round(ab[k] / (kstat[k]["avg_ms"] / 1e3) / 1e9, 1) for k in ab if k in kstat},
+            "results": results,
         }
         log("bench: FP%d %.1f Mcell-upd/s, %.3f ms/step, %s %.1f GB/s" % (
             prec, res[prec]["mcells"], res[prec]["ms_per_step"], dom, res[prec]["dom_gbs"]))
@@ -247,6 +293,7 @@ def main():
                        "dominant_gbs": round(r64["dom_gbs"], 1),
                        "kernels": {k: round(v["avg_ms"], 5) for k, v in r64["kernels"].items()}}
         out["fp32_over_fp64"] = round(r32["mcells"] / r64["mcells"], 3)
+        out["quality_f32_vs_f64"] = quality(r32["results"], r64["results"], case.freqs)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()
