@@ -84,6 +84,15 @@ SIGNATURES = {
     "cfd_model_port": ([_vp, C.c_int, _ip], C.c_int),
     "cfd_model_cells": ([_vp, C.c_int, _ip], C.c_int),
     "cfd_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
+    "cfd_batch_create": ([C.POINTER(_Config), C.c_int, C.POINTER(_vp)], C.c_int),
+    "cfd_batch_destroy": ([_vp], C.c_int),
+    "cfd_batch_size": ([_vp], C.c_int),
+    "cfd_batch_step": ([_vp, C.c_int], C.c_int),
+    "cfd_batch_sync": ([_vp], C.c_int),
+    "cfd_batch_member": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
+    "cfd_batch_device_bytes": ([_vp, C.POINTER(C.c_size_t)], C.c_int),
+    "cfd_batch_set_timing": ([_vp, C.c_int], C.c_int),
+    "cfd_batch_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
     "cfd_set_timing": ([_vp, C.c_int], C.c_int),
 }
 
@@ -159,11 +168,15 @@ def make_config(case, precision=CFD_F64):
 class Engine:
     """One device-resident simulation (handle of the C ABI)."""
 
-    def __init__(self, case, precision=CFD_F64):
+    def __init__(self, case, precision=CFD_F64, _handle=None):
         L = library()
-        cfg, self._keep = make_config(case, precision)
-        h = C.c_void_p()
-        _check(L.cfd_create(C.byref(cfg), C.byref(h)))
+        if _handle is None:
+            cfg, self._keep = make_config(case, precision)
+            h = C.c_void_p()
+            _check(L.cfd_create(C.byref(cfg), C.byref(h)))
+            self._owned = True
+        else:  # borrowed view of a batch member
+            h, self._keep, self._owned = _handle, None, False
         self.h = h
         self.precision = precision
         n = np.zeros(4, np.int32)
@@ -180,7 +193,8 @@ class Engine:
 
     def close(self):
         if getattr(self, "h", None):
-            library().cfd_destroy(self.h)
+            if getattr(self, "_owned", True):
+                library().cfd_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -283,13 +297,78 @@ class Engine:
         _check(library().cfd_set_timing(self.h, 1 if on else 0))
 
     def kernel_times(self):
-        cap = 64
-        names = (C.c_char * 32 * cap)()
-        ms = np.zeros(cap, np.float64)
-        cnt = np.zeros(cap, np.int32)
-        n = _check(library().cfd_kernel_times(self.h, C.cast(names, C.c_void_p), ms.ctypes.data_as(_dp),
-                                              cnt.ctypes.data_as(_ip), cap))
-        return {bytes(names[i]).split(b"\0")[0].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+        return _kernel_times(library().cfd_kernel_times, self.h)
+
+
+def _kernel_times(fn, h):
+    """{pass name: (total ms, launches)} from cfd_kernel_times / cfd_batch_kernel_times."""
+    cap = 64
+    names = (C.c_char * 32 * cap)()
+    ms = np.zeros(cap, np.float64)
+    cnt = np.zeros(cap, np.int32)
+    n = _check(fn(h, C.cast(names, C.c_void_p), ms.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip), cap))
+    return {bytes(names[i]).split(b"\0")[0].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+
+class Batch:
+    """Simulations of one shape stepped together: each pass of a step is one
+    launch over all instances (cfd_batch_*).  member(i) is a read-only Engine
+    view of instance i (fields, ports, gain populations)."""
+
+    def __init__(self, cases, precision=CFD_F64):
+        L = library()
+        cfgs = (_Config * len(cases))()
+        self._keep = []
+        for i, case in enumerate(cases):
+            cfg, keep = make_config(case, precision)
+            cfgs[i] = cfg
+            self._keep.append((cfg, keep))
+        h = C.c_void_p()
+        _check(L.cfd_batch_create(cfgs, len(cases), C.byref(h)))
+        self.h = h
+        self.cases = list(cases)
+        self.precision = precision
+        self._members = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self._members.values():
+                m.h = None
+            library().cfd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return _check(library().cfd_batch_size(self.h))
+
+    def step(self, n=1):
+        _check(library().cfd_batch_step(self.h, n))
+
+    def sync(self):
+        _check(library().cfd_batch_sync(self.h))
+
+    def member(self, i):
+        if i not in self._members:
+            h = C.c_void_p()
+            _check(library().cfd_batch_member(self.h, i, C.byref(h)))
+            self._members[i] = Engine(self.cases[i], self.precision, _handle=h)
+        return self._members[i]
+
+    def device_bytes(self):
+        v = C.c_size_t()
+        _check(library().cfd_batch_device_bytes(self.h, C.byref(v)))
+        return int(v.value)
+
+    def set_timing(self, on):
+        _check(library().cfd_batch_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        return _kernel_times(library().cfd_batch_kernel_times, self.h)
 
 
 class Model:

@@ -6,6 +6,8 @@ formats (so the reference's own parsers and the engine read the same files).
       TF/SF, two plane ports with a 64-frequency running DFT
   c3  192x192x64 disordered rod slab with four-level gain, periodic x/y,
       random-phase dipole, sphere port
+  c4  512 periodic 64x64x96 unit cells, one seeded scatterer each (a batch:
+      cpsfdtd_amd.Batch / cfd_batch_*)
 
 ``scale`` < 1 shrinks a workload isotropically for tests (cell counts, PML and
 box insets scale; physics parameters stay).
@@ -176,3 +178,52 @@ def c3(directory, scale=1.0, nt=20000, flags=FIXED, seed=1234, record=False):
     rs = _s(60, scale, 4)
     w.sphere_port("sphere.txt", rs * d[0], (nx // 2) * d[0], (ny // 2) * d[1], (nz // 2) * d[2], 70.0, 110.0, 0.0, 360.0)
     return w.run(nt, record_ports=record, periodic=3, flags=flags)
+
+
+C4_FREQS = [C0 / lam for lam in np.linspace(550e-9, 900e-9, 32)]
+
+
+def c4_params(n=512, seed=7):
+    """Per-instance scatterers from one MT19937_64(seed) stream, three draws
+    per instance in order: kind (a Drude disk with probability 0.3, else a
+    dielectric pillar), radius U(5, 25) cells, eps_r U(1.5, 12)."""
+    rng = MT19937_64(seed)
+    out = []
+    for _ in range(n):
+        uk, ur, ue = rng.uniform(), rng.uniform(), rng.uniform()
+        out.append(("drude" if uk < 0.3 else "dielectric", 5.0 + 20.0 * ur, 1.5 + 10.5 * ue))
+    return out
+
+
+def c4(directory, params, scale=1.0, nt=2000, flags=FIXED):
+    """One unit cell of the batched workload: 64x64x96, dx=10 nm, x and y
+    periodic, 10-cell CPML in z, a normal-incidence plane wave (TF/SF z faces
+    18 cells in, 700 nm, 6 fs FWHM, E along x); a centred z-axis scatterer
+    from ``params`` = (kind, radius in cells, eps_r): a dielectric pillar 16
+    cells tall, or a Drude disk 5 cells tall (wp 1.2e16, gamma 1e14 rad/s);
+    reflection and transmission plane ports over the whole cell with a running
+    DFT at 32 frequencies (550-900 nm in wavelength, every 4th step)."""
+    kind, radius, eps = params
+    n = _s(64, scale, 16)
+    nz = _s(96, scale, 40)
+    d = (10e-9, 10e-9, 10e-9)
+    npml = _s(10, scale, 4)
+    inset = npml + _s(8, scale, 3)
+    r = radius * scale
+    c = n // 2
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (eps, 1, 0, 0), (1, 1, 0, 0)])
+    h = _s(16, scale, 2) if kind == "dielectric" else _s(5, scale, 1)
+    z0 = nz // 2 - h // 2
+    w.grid((n, n, nz), d, [Shape(1 if kind == "dielectric" else 2, "cylinder", x0=c * d[0], y0=c * d[1], z0=z0 * d[2],
+                                 a=r * d[0], b=r * d[1], c=h * d[2])])
+    w.cpml(((0, 0), (0, 0), (npml, npml)), m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    lam = 700e-9
+    w.tfsf(lam, dlambda_for_fwhm(lam, 6e-15), 1e3, 0.0, c * d[0], c * d[1], 0.0, 0.0, 0.0, 1.0, 0.0,
+           ((0, 0), (0, 0), (inset, inset)), nt)
+    if kind == "drude":
+        w.drude_lorentz(1.0, [(1.2e16, 1e14)], [])
+    refl = npml + (inset - npml) // 2
+    w.plane_port_cells("port_r.txt", "z", refl, 0, n - 1, 0, n - 1, d)
+    w.plane_port_cells("port_t.txt", "z", nz - refl, 0, n - 1, 0, n - 1, d)
+    return w.run(nt, freqs=C4_FREQS, dft_stride=4, periodic=3, flags=flags)

@@ -73,11 +73,27 @@ struct KernelTimer {
 };
 
 // ----------------------------------------------------------------------------
+// FNV-1a over raw bytes: a batch checks that its instances share a geometry
+inline void sig_add(uint64_t &h, const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+}
+template <class T>
+inline void sig_add(uint64_t &h, const std::vector<T> &v) {
+    const size_t n = v.size();
+    sig_add(h, &n, sizeof n);
+    if (n) sig_add(h, v.data(), n * sizeof(T));
+}
+
 class SimBase {
 public:
     virtual ~SimBase() = default;
     Model m;
     int precision = 64;
+    bool member = false;     // instance of a batch: stepped by the batch only
+    bool own_stream = true;
+    uint64_t tf_sig = 14695981039346656037ull;    // TF/SF target lists, terms and waveforms
+    uint64_t port_sig = 14695981039346656037ull;  // port cells, frequencies, recording
     hipStream_t stream = nullptr;
     int done = 0;
     size_t dev_bytes = 0;
@@ -147,10 +163,15 @@ public:
         return (long long)i + (long long)d.px * ((long long)j + (long long)d.py * k);
     }
 
-    explicit Sim(Model &&model, int prec) {
+    explicit Sim(Model &&model, int prec, hipStream_t shared = nullptr) {
         m = std::move(model);
         precision = prec;
-        HIP_OK(hipStreamCreate(&stream));
+        if (shared) {
+            stream = shared;
+            own_stream = false;
+        } else {
+            HIP_OK(hipStreamCreate(&stream));
+        }
         try {
             setup();
         } catch (...) {
@@ -163,7 +184,7 @@ public:
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         allocs.clear();
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
 
@@ -295,6 +316,8 @@ public:
             tcoef = upload<double>(c); tdelay = upload<double>(dl); tprof = upload<double>(pr);
             einc = upload<double>(m.Einc); hinc = upload<double>(m.Hinc);
             tablen = (int)m.Einc.size();
+            sig_add(tf_sig, te); sig_add(tf_sig, th); sig_add(tf_sig, c); sig_add(tf_sig, dl); sig_add(tf_sig, pr);
+            sig_add(tf_sig, m.Einc); sig_add(tf_sig, m.Hinc);
         }
         // sources
         if (!m.src.empty()) {
@@ -323,7 +346,11 @@ public:
             dp.ipart = alloc<double>((size_t)m.nt * std::max(dp.nblk, 1));
             dports.push_back(dp);
             gather_max = std::max(gather_max, dp.npts);
+            sig_add(port_sig, P.ijk);
         }
+        sig_add(port_sig, m.freqs);
+        sig_add(port_sig, &m.dft_stride, sizeof m.dft_stride);
+        sig_add(port_sig, &m.record, sizeof m.record);
         if (gather_max) gather_buf = alloc<R>((size_t)gather_max * 6);
         setup_passes();
         if (!dports.empty()) {
@@ -464,17 +491,17 @@ public:
             const unsigned nb = (unsigned)(g.gx * g.gy * g.gz);
             timer.run(stream, h ? "full_h" : "full_e", [&] {
                 if (h) {
-                    if (leanH_cs) hipLaunchKernelGGL((k_full_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanH_st);
-                    else hipLaunchKernelGGL((k_full_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanH_st);
+                    if (leanH_cs) hipLaunchKernelGGL((k_full_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanH_st);
+                    else hipLaunchKernelGGL((k_full_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanH_st);
                 } else {
-                    if (leanE_cs) hipLaunchKernelGGL((k_full_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanE_st);
-                    else hipLaunchKernelGGL((k_full_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, g, L, leanE_st);
+                    if (leanE_cs) hipLaunchKernelGGL((k_full_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanE_st);
+                    else hipLaunchKernelGGL((k_full_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanE_st);
                 }
             });
             HIP_OK(hipGetLastError());
             if (!h && nspecial) {
                 timer.run(stream, "special_e", [&] {
-                    hipLaunchKernelGGL(k_special_e<R>, dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, special, nspecial);
+                    hipLaunchKernelGGL((k_special_e<R>), dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, nullptr, special, nspecial, nullptr);
                 });
                 HIP_OK(hipGetLastError());
             }
@@ -504,7 +531,7 @@ public:
         }
         if (!h && nspecial) {
             timer.run(stream, "special_e", [&] {
-                hipLaunchKernelGGL(k_special_e<R>, dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, special, nspecial);
+                hipLaunchKernelGGL((k_special_e<R>), dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, nullptr, special, nspecial, nullptr);
             });
             HIP_OK(hipGetLastError());
         }
@@ -525,8 +552,8 @@ public:
         const int nt = h ? ntfH : ntfE;
         if (!nt) return;
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
-            hipLaunchKernelGGL(k_tfsf<R>, dim3((nt + 255) / 256), dim3(256), 0, stream, d, h ? tfH : tfE, nt, tcoef,
-                               tdelay, tprof, h ? einc : hinc, tablen, n);
+            hipLaunchKernelGGL((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, h ? tfH : tfE, nt,
+                               tcoef, tdelay, tprof, h ? einc : hinc, tablen, n);
         });
         HIP_OK(hipGetLastError());
     }
@@ -548,7 +575,7 @@ public:
             slot = dft_pending++;
         }
         timer.run(stream, "ports", [&] {
-            hipLaunchKernelGGL(k_port<R>, dim3(port_blocks), dim3(256), 0, stream, d, port_tab, (int)dports.size(), n, slot);
+            hipLaunchKernelGGL((k_port<R>), dim3(port_blocks), dim3(256), 0, stream, d, nullptr, port_tab, (int)dports.size(), n, slot);
         });
         HIP_OK(hipGetLastError());
         if (dft_pending == kDftBatch) dft_flush();
@@ -558,7 +585,7 @@ public:
         const int nb = dft_pending;
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
-            hipLaunchKernelGGL(k_dft<R>, dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
+            hipLaunchKernelGGL((k_dft<R>), dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
                                dft_first, m.dft_stride, twE, twH, (int)m.freqs.size(), m.nt);
         });
         HIP_OK(hipGetLastError());
@@ -657,6 +684,181 @@ public:
     }
 };
 
+// ----------------------------------------------------------------------------
+// A batch of simulations of one shape (grid, CPML, TF/SF box and waveform,
+// ports) whose media and geometry may differ: every pass of a step is one
+// launch over all instances (blockIdx.y = instance), on the batch's stream.
+// Each instance is an ordinary Sim (same set-up, same readers) that shares
+// that stream; the batch owns its stepping.
+class BatchBase {
+public:
+    virtual ~BatchBase() = default;
+    hipStream_t stream = nullptr;
+    int done = 0;
+    KernelTimer timer;
+    virtual int size() const = 0;
+    virtual SimBase *member(int i) = 0;
+    virtual void step(int count) = 0;
+    virtual size_t device_bytes() const = 0;
+};
+
+template <class R>
+class Batch : public BatchBase {
+public:
+    std::vector<std::unique_ptr<Sim<R>>> sims;
+    std::vector<void *> allocs;
+    Dev<R> *devs = nullptr;
+    SpecialList *lists = nullptr;
+    int max_special = 0;
+    PortDesc<R> *ports_all = nullptr;  // [instance][port]
+    int nports = 0;
+    int dft_pending = 0, dft_first = 0;
+    size_t own_bytes = 0;
+
+    Batch(std::vector<Model> &&models, int prec) {
+        if (models.empty()) throw Error("empty batch");
+        HIP_OK(hipStreamCreate(&stream));
+        try {
+            for (Model &m : models) {
+                sims.emplace_back(new Sim<R>(std::move(m), prec, stream));
+                sims.back()->member = true;
+            }
+            check_shape();
+            std::vector<Dev<R>> dv;
+            std::vector<SpecialList> sl;
+            std::vector<PortDesc<R>> pa;
+            for (auto &sp : sims) {
+                dv.push_back(sp->d);
+                sl.push_back(SpecialList{sp->special, sp->nspecial, 0});
+                max_special = std::max(max_special, sp->nspecial);
+                pa.insert(pa.end(), sp->dports.begin(), sp->dports.end());
+            }
+            nports = (int)sims[0]->dports.size();
+            devs = upload(dv);
+            lists = upload(sl);
+            if (nports) ports_all = upload(pa);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~Batch() override { release(); }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        sims.clear();
+        for (void *p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    template <class T>
+    T *upload(const std::vector<T> &v) {
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)));
+        allocs.push_back(p);
+        own_bytes += v.size() * sizeof(T);
+        HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return static_cast<T *>(p);
+    }
+    void check_shape() {
+        const Sim<R> &a = *sims[0];
+        for (size_t i = 1; i < sims.size(); i++) {
+            const Sim<R> &b = *sims[i];
+            const std::string who = "batch instance " + std::to_string(i) + ": ";
+            if (b.m.Nx != a.m.Nx || b.m.Ny != a.m.Ny || b.m.Nz != a.m.Nz) throw Error(who + "grid size differs from instance 0");
+            if (std::memcmp(b.m.npml, a.m.npml, sizeof a.m.npml) || b.m.pbx != a.m.pbx || b.m.pby != a.m.pby)
+                throw Error(who + "CPML or periodic axes differ from instance 0");
+            if (b.m.nt != a.m.nt || b.m.dt != a.m.dt) throw Error(who + "time step or run length differs from instance 0");
+            if (b.m.has_tfsf != a.m.has_tfsf || b.tf_sig != a.tf_sig) throw Error(who + "TF/SF set-up differs from instance 0");
+            if (b.dports.size() != a.dports.size() || b.port_sig != a.port_sig) throw Error(who + "ports differ from instance 0");
+            if (b.leanH_cs != a.leanH_cs || b.leanE_cs != a.leanE_cs ||
+                std::memcmp(&b.leanH_st, &a.leanH_st, sizeof a.leanH_st) || std::memcmp(&b.leanE_st, &a.leanE_st, sizeof a.leanE_st))
+                throw Error(who + "interior stretch factors differ from instance 0");
+        }
+    }
+    int size() const override { return (int)sims.size(); }
+    SimBase *member(int i) override { return sims.at(i).get(); }
+    size_t device_bytes() const override {
+        size_t b = own_bytes;
+        for (const auto &sp : sims) b += sp->dev_bytes;
+        return b;
+    }
+
+    void step(int count) override {
+        const Sim<R> &a = *sims[0];
+        const unsigned ni = (unsigned)sims.size();
+        const TileGrid g = a.tiles();
+        const unsigned nb = (unsigned)(g.gx * g.gy * g.gz);
+        const Dev<R> d0 = a.d;  // by-value argument, unused by the batched kernels
+        for (int c = 0; c < count; c++) {
+            const int n = done;
+            if (n >= a.m.nt) throw Error("run finished: all " + std::to_string(a.m.nt) + " steps done");
+            timer.run(stream, "full_h", [&] {
+                if (a.leanH_cs) hipLaunchKernelGGL((k_full_h<R, true, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanH, a.leanH_st);
+                else hipLaunchKernelGGL((k_full_h<R, false, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanH, a.leanH_st);
+            });
+            HIP_OK(hipGetLastError());
+            tfsf(true, n);
+            timer.run(stream, "full_e", [&] {
+                if (a.leanE_cs) hipLaunchKernelGGL((k_full_e<R, true, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanE, a.leanE_st);
+                else hipLaunchKernelGGL((k_full_e<R, false, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanE, a.leanE_st);
+            });
+            HIP_OK(hipGetLastError());
+            if (max_special) {
+                timer.run(stream, "special_e", [&] {
+                    hipLaunchKernelGGL((k_special_e<R, true>), dim3((max_special + 255) / 256, ni), dim3(256), 0, stream, d0, devs,
+                                       nullptr, 0, lists);
+                });
+                HIP_OK(hipGetLastError());
+            }
+            tfsf(false, n);
+            for (auto &sp : sims) sp->sources(n);  // instances with point sources (rare in a batch)
+            ports(n);
+            done++;
+            for (auto &sp : sims) sp->done++;
+        }
+        dft_flush();  // members' DFT readers then see every sampled step
+    }
+    void tfsf(bool h, int n) {
+        const Sim<R> &a = *sims[0];
+        if (!a.m.has_tfsf) return;
+        const int nt = h ? a.ntfH : a.ntfE;
+        if (!nt) return;
+        timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
+            hipLaunchKernelGGL((k_tfsf<R, true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
+                               h ? a.tfH : a.tfE, nt, a.tcoef, a.tdelay, a.tprof, h ? a.einc : a.hinc, a.tablen, n);
+        });
+        HIP_OK(hipGetLastError());
+    }
+    void ports(int n) {
+        const Sim<R> &a = *sims[0];
+        if (!nports) return;
+        int slot = -1;
+        if (!a.m.freqs.empty() && n % a.m.dft_stride == 0) {
+            if (dft_pending == 0) dft_first = n;
+            slot = dft_pending++;
+        }
+        timer.run(stream, "ports", [&] {
+            hipLaunchKernelGGL((k_port<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
+                               ports_all, nports, n, slot);
+        });
+        HIP_OK(hipGetLastError());
+        if (dft_pending == Sim<R>::kDftBatch) dft_flush();
+    }
+    void dft_flush() {
+        if (!dft_pending) return;
+        const Sim<R> &a = *sims[0];
+        const int nbatch = dft_pending;
+        dft_pending = 0;
+        timer.run(stream, "port_dft", [&] {
+            hipLaunchKernelGGL((k_dft<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, ports_all,
+                               nports, nbatch, dft_first, a.m.dft_stride, a.twE, a.twH, (int)a.m.freqs.size(), a.m.nt);
+        });
+        HIP_OK(hipGetLastError());
+    }
+};
+
 }  // namespace cfd
 
 // ============================================================================
@@ -665,7 +867,13 @@ public:
 using cfd::SimBase;
 
 struct cfd_sim {
-    std::unique_ptr<SimBase> impl;
+    std::unique_ptr<SimBase> owned;  // null for a batch member
+    SimBase *impl = nullptr;
+};
+
+struct cfd_batch {
+    std::unique_ptr<cfd::BatchBase> impl;
+    std::vector<std::unique_ptr<cfd_sim>> members;  // borrowed views of the instances
 };
 
 template <class F>
@@ -688,6 +896,16 @@ static int guard(F &&f)
             cfd::g_error = "null simulation handle";     \
             return -1;                                   \
         }                                                \
+    } while (0)
+
+// calls that advance or update a simulation: not on a batch member
+#define NEED_SOLO(s)                                                                  \
+    do {                                                                              \
+        NEED(s);                                                                      \
+        if ((s)->impl->member) {                                                      \
+            cfd::g_error = "batch member: the batch steps its instances (cfd_batch_step)"; \
+            return -1;                                                                \
+        }                                                                             \
     } while (0)
 
 extern "C" {
@@ -715,8 +933,9 @@ int cfd_create(const cfd_config *cfg, cfd_sim **out)
         if (cfg->precision != CFD_F32 && cfg->precision != CFD_F64) throw cfd::Error("precision must be 32 or 64");
         cfd::Model m = cfd::build_model(*cfg);
         auto s = std::make_unique<cfd_sim>();
-        if (cfg->precision == CFD_F32) s->impl = std::make_unique<cfd::Sim<float>>(std::move(m), 32);
-        else s->impl = std::make_unique<cfd::Sim<double>>(std::move(m), 64);
+        if (cfg->precision == CFD_F32) s->owned = std::make_unique<cfd::Sim<float>>(std::move(m), 32);
+        else s->owned = std::make_unique<cfd::Sim<double>>(std::move(m), 64);
+        s->impl = s->owned.get();
         *out = s.release();
     });
 }
@@ -724,6 +943,7 @@ int cfd_create(const cfd_config *cfg, cfd_sim **out)
 int cfd_destroy(cfd_sim *s)
 {
     if (!s) return 0;
+    if (!s->owned) { cfd::g_error = "a batch member is released with its batch (cfd_batch_destroy)"; return -1; }
     delete s;
     return 0;
 }
@@ -745,7 +965,7 @@ int cfd_info(cfd_sim *s, int *nxyz_nt, double *dd, int *counts)
 
 size_t cfd_device_bytes(const cfd_sim *s) { return (s && s->impl) ? s->impl->dev_bytes : 0; }
 
-int cfd_step(cfd_sim *s, int count) { NEED(s); return guard([&] { s->impl->step(count); }); }
+int cfd_step(cfd_sim *s, int count) { NEED_SOLO(s); return guard([&] { s->impl->step(count); }); }
 int cfd_steps_done(cfd_sim *s) { NEED(s); return s->impl->done; }
 int cfd_sync(cfd_sim *s)
 {
@@ -755,19 +975,19 @@ int cfd_sync(cfd_sim *s)
         s->impl->timer.collect();
     });
 }
-int cfd_basic_h(cfd_sim *s) { NEED(s); return guard([&] { s->impl->basic_h(); }); }
-int cfd_basic_e(cfd_sim *s) { NEED(s); return guard([&] { s->impl->basic_e(); }); }
-int cfd_cpml_h(cfd_sim *s) { NEED(s); return guard([&] { s->impl->cpml_h(); }); }
-int cfd_cpml_e(cfd_sim *s) { NEED(s); return guard([&] { s->impl->cpml_e(); }); }
-int cfd_tfsf_h(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->tfsf(true, n); }); }
-int cfd_tfsf_e(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->tfsf(false, n); }); }
-int cfd_dl_e(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_dl) s->impl->e_parts(cfd::E_DL); }); }
-int cfd_gain_pol(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOL); }); }
-int cfd_gain_e(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINE); }); }
-int cfd_gain_pop(cfd_sim *s) { NEED(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOP); }); }
-int cfd_sources(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->sources(n); }); }
-int cfd_ports_sample(cfd_sim *s, int n) { NEED(s); return guard([&] { s->impl->ports(n); }); }
-int cfd_advance(cfd_sim *s) { NEED(s); s->impl->done++; return 0; }
+int cfd_basic_h(cfd_sim *s) { NEED_SOLO(s); return guard([&] { s->impl->basic_h(); }); }
+int cfd_basic_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { s->impl->basic_e(); }); }
+int cfd_cpml_h(cfd_sim *s) { NEED_SOLO(s); return guard([&] { s->impl->cpml_h(); }); }
+int cfd_cpml_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { s->impl->cpml_e(); }); }
+int cfd_tfsf_h(cfd_sim *s, int n) { NEED_SOLO(s); return guard([&] { s->impl->tfsf(true, n); }); }
+int cfd_tfsf_e(cfd_sim *s, int n) { NEED_SOLO(s); return guard([&] { s->impl->tfsf(false, n); }); }
+int cfd_dl_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_dl) s->impl->e_parts(cfd::E_DL); }); }
+int cfd_gain_pol(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOL); }); }
+int cfd_gain_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINE); }); }
+int cfd_gain_pop(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOP); }); }
+int cfd_sources(cfd_sim *s, int n) { NEED_SOLO(s); return guard([&] { s->impl->sources(n); }); }
+int cfd_ports_sample(cfd_sim *s, int n) { NEED_SOLO(s); return guard([&] { s->impl->ports(n); }); }
+int cfd_advance(cfd_sim *s) { NEED_SOLO(s); s->impl->done++; return 0; }
 
 int cfd_read_field(cfd_sim *s, int comp, double *host)
 {
@@ -832,6 +1052,89 @@ int cfd_kernel_times(cfd_sim *s, char (*names)[32], double *ms, int *launches, i
         HIP_OK(hipStreamSynchronize(s->impl->stream));
         s->impl->timer.collect();
         for (auto &kv : s->impl->timer.acc) {
+            if (n >= cap) break;
+            snprintf(names[n], 32, "%s", kv.first.c_str());
+            ms[n] = kv.second.first;
+            launches[n] = kv.second.second;
+            n++;
+        }
+    });
+    return rc ? rc : n;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+int cfd_batch_create(const cfd_config *cfgs, int n, cfd_batch **out)
+{
+    if (!cfgs || !out || n <= 0) { cfd::g_error = "null argument or empty batch"; return -1; }
+    *out = nullptr;
+    return guard([&] {
+        const int prec = cfgs[0].precision;
+        if (prec != CFD_F32 && prec != CFD_F64) throw cfd::Error("precision must be 32 or 64");
+        std::vector<cfd::Model> models;
+        for (int i = 0; i < n; i++) {
+            if (cfgs[i].precision != prec) throw cfd::Error("batch instances must share one precision");
+            models.push_back(cfd::build_model(cfgs[i]));
+        }
+        auto b = std::make_unique<cfd_batch>();
+        if (prec == CFD_F32) b->impl = std::make_unique<cfd::Batch<float>>(std::move(models), 32);
+        else b->impl = std::make_unique<cfd::Batch<double>>(std::move(models), 64);
+        for (int i = 0; i < n; i++) {
+            auto m = std::make_unique<cfd_sim>();
+            m->impl = b->impl->member(i);
+            b->members.push_back(std::move(m));
+        }
+        *out = b.release();
+    });
+}
+int cfd_batch_destroy(cfd_batch *b)
+{
+    if (!b) return 0;
+    b->members.clear();
+    delete b;
+    return 0;
+}
+#define NEED_BATCH(b)                                   \
+    do {                                                \
+        if (!(b) || !(b)->impl) {                       \
+            cfd::g_error = "null batch handle";         \
+            return -1;                                  \
+        }                                               \
+    } while (0)
+int cfd_batch_size(cfd_batch *b) { NEED_BATCH(b); return b->impl->size(); }
+int cfd_batch_step(cfd_batch *b, int count) { NEED_BATCH(b); return guard([&] { b->impl->step(count); }); }
+int cfd_batch_sync(cfd_batch *b) { NEED_BATCH(b); return guard([&] { HIP_OK(hipStreamSynchronize(b->impl->stream)); }); }
+int cfd_batch_member(cfd_batch *b, int i, cfd_sim **out)
+{
+    NEED_BATCH(b);
+    if (!out || i < 0 || i >= (int)b->members.size()) { cfd::g_error = "no such batch member"; return -1; }
+    *out = b->members[i].get();
+    return 0;
+}
+int cfd_batch_device_bytes(cfd_batch *b, size_t *bytes)
+{
+    NEED_BATCH(b);
+    if (bytes) *bytes = b->impl->device_bytes();
+    return 0;
+}
+int cfd_batch_set_timing(cfd_batch *b, int on)
+{
+    NEED_BATCH(b);
+    return guard([&] {
+        HIP_OK(hipStreamSynchronize(b->impl->stream));
+        b->impl->timer.collect();
+        b->impl->timer.acc.clear();
+        b->impl->timer.on = on != 0;
+    });
+}
+int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *launches, int cap)
+{
+    NEED_BATCH(b);
+    int n = 0;
+    int rc = guard([&] {
+        HIP_OK(hipStreamSynchronize(b->impl->stream));
+        b->impl->timer.collect();
+        for (auto &kv : b->impl->timer.acc) {
             if (n >= cap) break;
             snprintf(names[n], 32, "%s", kv.first.c_str());
             ms[n] = kv.second.first;

@@ -524,6 +524,16 @@ __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
     d.F[2][p] = E[2];
 }
 
+// Batched launches: one launch covers every instance of a batch (gridDim.y
+// instances, blockIdx.y = instance); the instance's view is read from the
+// device table devs instead of the by-value kernel argument.
+template <class R, bool B>
+__device__ __forceinline__ const Dev<R> &instance(const Dev<R> &d0, const Dev<R> *devs)
+{
+    if constexpr (B) return devs[blockIdx.y];
+    else return d0;
+}
+
 // ---------------------------------------------------------------------------
 // Merged half-step kernels: one launch over every tile of the update range.
 // A tile inside the interior box L runs the lean stencil; any other tile runs
@@ -546,9 +556,10 @@ __device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, int x0, int y
     return m;
 }
 
-template <class R, bool CS>
-__global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d, TileGrid g, Box L, LeanStretch<R> c)
+template <class R, bool CS, bool B = false>
+__global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs, TileGrid g, Box L, LeanStretch<R> c)
 {
+    const Dev<R> &d = instance<R, B>(d0, devs);
     int bx, by, k;
     if (!tile_of(g, bx, by, k)) return;
     const int x0 = bx * TX, y0 = by * TY;
@@ -577,9 +588,10 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d, TileGrid g, Box L, 
     d.F[5][p] = hz;
 }
 
-template <class R, bool CS>
-__global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d, TileGrid g, Box L, LeanStretch<R> c)
+template <class R, bool CS, bool B = false>
+__global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs, TileGrid g, Box L, LeanStretch<R> c)
 {
+    const Dev<R> &d = instance<R, B>(d0, devs);
     int bx, by, k;
     if (!tile_of(g, bx, by, k)) return;
     const int x0 = bx * TX, y0 = by * TY;
@@ -617,9 +629,17 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d, TileGrid g, Box L, 
 // kSpecialPml set for a cell outside the interior box (its CPML slab terms
 // follow the ADE, as in the full update).
 constexpr int kSpecialPml = 62;
-template <class R>
-__global__ __launch_bounds__(256) void k_special_e(Dev<R> d, const long long *cells, int n)
+struct SpecialList {
+    const long long *cells;
+    int n, pad;
+};
+template <class R, bool B = false>
+__global__ __launch_bounds__(256) void k_special_e(Dev<R> d0, const Dev<R> *devs, const long long *cells0, int n0,
+                                                   const SpecialList *lists)
 {
+    const Dev<R> &d = instance<R, B>(d0, devs);
+    const long long *cells = B ? lists[blockIdx.y].cells : cells0;
+    const int n = B ? lists[blockIdx.y].n : n0;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
     const long long v = cells[t];
@@ -648,11 +668,12 @@ struct TfTarget {
 // TF/SF face corrections (TF_SF.cpp:988-1089 for E, :1201-1301 for H): one
 // thread per corrected field value, its face terms applied in the reference's
 // face order.  Incident values are interpolated in FP64.
-template <class R>
-__global__ __launch_bounds__(256) void k_tfsf(Dev<R> d, const TfTarget *tg, int ntg, const double *coef,
-                                              const double *delay, const double *prof, const double *tab, int tablen,
-                                              int n)
+template <class R, bool B = false>
+__global__ __launch_bounds__(256) void k_tfsf(Dev<R> d0, const Dev<R> *devs, const TfTarget *tg, int ntg,
+                                              const double *coef, const double *delay, const double *prof,
+                                              const double *tab, int tablen, int n)
 {
+    const Dev<R> &d = instance<R, B>(d0, devs);
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= ntg) return;
     const TfTarget g = tg[t];
@@ -701,9 +722,12 @@ struct PortDesc {
 // Port gather (Ports.cpp:88-99, 342-353) into the time-series record and the
 // DFT batch, and the per-step sum of |E|^2 (Ports.cpp:512-522) reduced across
 // the workgroup with wavefront shuffles into one partial per workgroup.
-template <class R>
-__global__ __launch_bounds__(256) void k_port(Dev<R> d, const PortDesc<R> *ports, int nports, int n, int slot)
+template <class R, bool B = false>
+__global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, const PortDesc<R> *ports, int nports,
+                                              int n, int slot)
 {
+    const Dev<R> &d = instance<R, B>(d0, devs);
+    if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     __shared__ double red[4];
     int q = 0;
     while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
@@ -732,10 +756,11 @@ __global__ __launch_bounds__(256) void k_port(Dev<R> d, const PortDesc<R> *ports
 // accumulator is read once, gets the batch's terms in step order (the same
 // additions, in the same order, as accumulating every sampled step), and is
 // written once.  E samples carry phase 2 pi f (n+1) dt, H samples (n+1/2) dt.
-template <class R>
+template <class R, bool B = false>
 __global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nports, int nb, int n0, int stride,
                                              const double *twE, const double *twH, int nf, int ntot)
 {
+    if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     int q = 0;
     while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
     const PortDesc<R> P = ports[q];

@@ -109,6 +109,25 @@ int cfd_port_dft(cfd_sim *s, int port, double *host);           /* [npts][n_freq
 int cfd_port_intensity(cfd_sim *s, int port, double *host);     /* [steps] sum of |E|^2 */
 int cfd_gain_read(cfd_sim *s, double *host);                    /* [cells][4] N0..N3 */
 
+/* Batches: n simulations of one shape (grid, CPML, periodic axes, TF/SF box
+ * and waveform, ports, run length, precision) whose media, geometry and
+ * dispersive cells may differ.  Each pass of a step is one launch over all
+ * instances.  No reference counterpart (the reference runs one simulation
+ * per process, main.cpp:115-343); each instance behaves as cfd_create of its
+ * config followed by cfd_step, bit for bit. */
+typedef struct cfd_batch cfd_batch;
+int cfd_batch_create(const cfd_config *cfgs, int n, cfd_batch **out);
+int cfd_batch_destroy(cfd_batch *b);
+int cfd_batch_size(cfd_batch *b);
+int cfd_batch_step(cfd_batch *b, int count);                 /* main.cpp:267-343 for every instance */
+int cfd_batch_sync(cfd_batch *b);
+/* a borrowed handle of instance i for the readers (cfd_read_field,
+ * cfd_port_*, cfd_gain_read ...); it cannot be stepped or destroyed */
+int cfd_batch_member(cfd_batch *b, int i, cfd_sim **out);
+int cfd_batch_device_bytes(cfd_batch *b, size_t *bytes);
+int cfd_batch_set_timing(cfd_batch *b, int on);
+int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *launches, int cap);
+
 /* Host-side set-up only, no device: the parsed model the engine uploads
  * (Structure, CPML, TF_SF, port constructors).  Usable without a GPU. */
 typedef struct cfd_model cfd_model;
