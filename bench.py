@@ -186,6 +186,41 @@ def cpu_baseline(scale_steps=200):
             "sample": "3 steps of the same workload on the FP64 oracle, %.1f s" % sec}
 
 
+def batched_c4(steps=200, warmup=10, n=512):
+    """The batched unit-cell workload: n periodic 64x64x96 cells (seeded
+    pillars and Drude disks, normal-incidence TF/SF, R/T ports with a
+    32-frequency DFT), FP32, one launch per pass over all cells."""
+    import cpsfdtd_amd as eng
+    from cpsfdtd_amd import configs
+    root = tempfile.mkdtemp(prefix="bench_c4_")
+    ps = configs.c4_params(n)
+    cases = []
+    for i in range(n):
+        d = os.path.join(root, "i%d" % i)
+        os.makedirs(d)
+        cases.append(configs.c4(d, ps[i], nt=steps + warmup + 20))
+    t0 = time.perf_counter()
+    b = eng.Batch(cases, 32)
+    setup = time.perf_counter() - t0
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    b.set_timing(True)
+    b.step(20)
+    b.sync()
+    kt = b.kernel_times()
+    nbytes = b.device_bytes()
+    b.close()
+    cells = n * 64 * 64 * 96
+    return {"workload": "c4: %d periodic 64x64x96 cells, FP32" % n, "instances": n, "steps": steps,
+            "value": round(cells * steps / wall / 1e6, 2), "unit": "Mcell-updates/s",
+            "ms_per_step": round(wall / steps * 1e3, 4), "setup_s": round(setup, 2), "device_gb": round(nbytes / 1e9, 2),
+            "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +229,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched c4 measurement")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -294,6 +330,10 @@ def main():
                        "kernels": {k: round(v["avg_ms"], 5) for k, v in r64["kernels"].items()}}
         out["fp32_over_fp64"] = round(r32["mcells"] / r64["mcells"], 3)
         out["quality_f32_vs_f64"] = quality(r32["results"], r64["results"], case.freqs)
+    if not args.no_batch:
+        log("bench: batched c4 (512 unit cells)")
+        out["batched"] = batched_c4()
+        log("bench: batched c4 %.1f Mcell-upd/s" % out["batched"]["value"])
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()
