@@ -94,6 +94,17 @@ SIGNATURES = {
     "cfd_batch_set_timing": ([_vp, C.c_int], C.c_int),
     "cfd_batch_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
     "cfd_set_timing": ([_vp, C.c_int], C.c_int),
+    "cfd_query": ([_vp], C.c_int),
+    "cfd_server_create": ([C.c_void_p, C.POINTER(_vp)], C.c_int),
+    "cfd_server_destroy": ([_vp], C.c_int),
+    "cfd_server_submit": ([_vp, C.POINTER(_Config), C.POINTER(C.c_longlong)], C.c_int),
+    "cfd_server_poll": ([_vp, C.c_longlong, _dp], C.c_int),
+    "cfd_server_wait": ([_vp, C.c_longlong, C.c_double], C.c_int),
+    "cfd_server_result": ([_vp, C.c_longlong, C.c_int, C.c_int, _dp, C.POINTER(C.c_size_t)], C.c_int),
+    "cfd_server_request_error": ([_vp, C.c_longlong], C.c_char_p),
+    "cfd_server_release": ([_vp, C.c_longlong], C.c_int),
+    "cfd_server_stats": ([_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_size_t), _ip, _ip], C.c_int),
+    "cfd_server_last_error": ([], C.c_char_p),
 }
 
 _lib = None
@@ -369,6 +380,81 @@ class Batch:
 
     def kernel_times(self):
         return _kernel_times(library().cfd_batch_kernel_times, self.h)
+
+
+class _ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("quantum", C.c_int)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class Server:
+    """Serving loop (cfd_server_*): submit cases, then poll or wait; results
+    are the port |E|^2 series and DFTs, on the host once a request is done."""
+
+    def __init__(self, hbm_budget=0, max_active=0, quantum=0):
+        o = _ServerOpts(hbm_budget, max_active, quantum)
+        h = C.c_void_p()
+        if library().cfd_server_create(C.byref(o), C.byref(h)) < 0:
+            raise EngineError(library().cfd_server_last_error().decode())
+        self.h = h
+        self.cases = {}
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise EngineError(library().cfd_server_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().cfd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, case, precision=CFD_F64):
+        cfg, keep = make_config(case, precision)
+        rid = C.c_longlong()
+        self._chk(library().cfd_server_submit(self.h, C.byref(cfg), C.byref(rid)))
+        self.cases[rid.value] = case
+        return rid.value
+
+    def poll(self, rid):
+        t = np.zeros(3, np.float64)
+        st = self._chk(library().cfd_server_poll(self.h, rid, t.ctypes.data_as(_dp)))
+        return st, tuple(float(x) for x in t)
+
+    def wait(self, rid, timeout=600.0):
+        return self._chk(library().cfd_server_wait(self.h, rid, timeout))
+
+    def error(self, rid):
+        return library().cfd_server_request_error(self.h, rid).decode()
+
+    def result(self, rid, port, what="intensity"):
+        w = 1 if what == "dft" else 0
+        n = C.c_size_t()
+        self._chk(library().cfd_server_result(self.h, rid, port, w, None, C.byref(n)))
+        a = np.zeros(n.value, np.float64)
+        self._chk(library().cfd_server_result(self.h, rid, port, w, a.ctypes.data_as(_dp), C.byref(n)))
+        if w:
+            nf = len(self.cases[rid].freqs)
+            a = a.reshape(-1, nf, 6, 2)
+            return a[..., 0] + 1j * a[..., 1]
+        return a
+
+    def release(self, rid):
+        self._chk(library().cfd_server_release(self.h, rid))
+        self.cases.pop(rid, None)
+
+    def stats(self):
+        st, pb, pa, q = C.c_longlong(), C.c_size_t(), C.c_int(), C.c_int()
+        self._chk(library().cfd_server_stats(self.h, C.byref(st), C.byref(pb), C.byref(pa), C.byref(q)))
+        return {"steps_launched": st.value, "peak_bytes": pb.value, "peak_active": pa.value, "queued": q.value}
 
 
 class Model:

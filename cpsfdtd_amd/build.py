@@ -26,7 +26,7 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-I" + os.path.join
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 
-HOST_SOURCES = ["model.cpp", "model_capi.cpp"]
+HOST_SOURCES = ["model.cpp", "model_capi.cpp", "server.cpp"]
 HIP_SOURCES = ["engine.hip"]
 HEADERS = ["model.hpp", "engine.hpp", "kernels.hip"]
 
@@ -49,14 +49,14 @@ def build(verbose=False, force=True):
     objs = []
     for f in HOST_SOURCES:
         o = os.path.join(OBJ, f + ".o")
-        _run(["g++"] + COMMON + ["-Wall", "-c", os.path.join(CSRC, f), "-o", o], verbose)
+        _run(["g++"] + COMMON + ["-Wall", "-pthread", "-c", os.path.join(CSRC, f), "-o", o], verbose)
         objs.append(o)
     for f in HIP_SOURCES:
         o = os.path.join(OBJ, f + ".o")
         _run([HIPCC] + COMMON + HIP_FLAGS + ["-c", os.path.join(CSRC, f), "-o", o], verbose)
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
     build_facade_driver(verbose)
     return LIB

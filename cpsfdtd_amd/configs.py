@@ -8,6 +8,8 @@ formats (so the reference's own parsers and the engine read the same files).
       random-phase dipole, sphere port
   c4  512 periodic 64x64x96 unit cells, one seeded scatterer each (a batch:
       cpsfdtd_amd.Batch / cfd_batch_*)
+  c5  a request stream for the server (cpsfdtd_amd.Server / cfd_server_*):
+      Poisson arrivals, log-uniform grid edges, mixed physics
 
 ``scale`` < 1 shrinks a workload isotropically for tests (cell counts, PML and
 box insets scale; physics parameters stay).
@@ -227,3 +229,63 @@ def c4(directory, params, scale=1.0, nt=2000, flags=FIXED):
     w.plane_port_cells("port_r.txt", "z", refl, 0, n - 1, 0, n - 1, d)
     w.plane_port_cells("port_t.txt", "z", nz - refl, 0, n - 1, 0, n - 1, d)
     return w.run(nt, freqs=C4_FREQS, dft_stride=4, periodic=3, flags=flags)
+
+
+def c5_requests(n=2000, seed=99, rate=20.0):
+    """The serving workload's request stream from one MT19937_64(seed), six
+    draws per request in order: inter-arrival time (exponential, ``rate``
+    per second), grid edges Nx, Ny, Nz (each log-uniform over 32-320 cells),
+    steps (uniform integer 500-8000), kind (40 % dielectric sphere, 30 %
+    Drude-Lorentz sphere, 30 % gain slab).  Returns [(arrival_s, params)]."""
+    rng = MT19937_64(seed)
+    t = 0.0
+    out = []
+    lo, hi = math.log(32.0), math.log(320.0)
+    for _ in range(n):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        nx, ny, nz = (int(round(math.exp(lo + (hi - lo) * rng.uniform()))) for _ in range(3))
+        steps = 500 + int(rng.uniform() * 7501)
+        u = rng.uniform()
+        kind = "sphere" if u < 0.4 else "dl" if u < 0.7 else "gain"
+        out.append((t, {"N": (nx, ny, nz), "steps": steps, "kind": kind}))
+    return out
+
+
+def c5(directory, params, flags=FIXED):
+    """One request: dx = 10 nm, 10-cell CPML on every face; ``sphere``: an
+    eps_r 2.25 sphere (radius a fifth of the smallest edge) around a central
+    Ez point source; ``dl``: a Drude-Lorentz sphere of the c2 medium under a
+    normal-incidence plane wave (TF/SF box 11 cells in); ``gain``: a c3 gain
+    slab a quarter of the height thick with a central Ez point source.  A
+    plane port 11 cells below the top face with a running DFT at 8
+    frequencies (400-800 nm, every 8th step)."""
+    nx, ny, nz = params["N"]
+    nt = params["steps"]
+    kind = params["kind"]
+    d = (10e-9, 10e-9, 10e-9)
+    c = (nx // 2, ny // 2, nz // 2)
+    r = max(2, min(nx, ny, nz) // 5)
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0), (2.25, 1, 0, 0)])
+    if kind == "gain":
+        h = max(2, nz // 4)
+        shapes = [Shape(5, "cuboid", x0=0.0, y0=0.0, z0=(c[2] - h // 2) * d[2], a=(nx + 1) * d[0], b=(ny + 1) * d[1],
+                        c=h * d[2])]
+    else:
+        shapes = [Shape(1 if kind == "sphere" else 2, "ellipsoid", x0=c[0] * d[0], y0=c[1] * d[1], z0=c[2] * d[2],
+                        a=r * d[0], b=r * d[1], c=r * d[2])]
+    w.grid((nx, ny, nz), d, shapes)
+    w.cpml(10, m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.source(600e-9, 10e-15, 30e-15)
+    if kind == "dl":
+        lam = 600e-9
+        w.tfsf(lam, dlambda_for_fwhm(lam, 10e-15), 1e3, 0.0, c[0] * d[0], c[1] * d[1], 0.0, 0.0, 0.0, 1.0, 0.0,
+               ((11, 11), (11, 11), (11, 11)), nt)
+        w.drude_lorentz(3.7, [(1.37e16, 2.7e13)], [(4.2e15, 1e15, 1.0), (6.1e15, 2e15, 0.5)])
+    else:
+        w.point_source(c[0], c[1], c[2], comp=2, amp=1.0)
+    if kind == "gain":
+        w.gain(600e-9, 1e14, 1e-13, 1e-10, 1e-13, 3.3e24, 1e9)
+    w.plane_port_cells("port.txt", "z", nz - 11, 11, nx - 12, 11, ny - 12, d)
+    freqs = [C0 / lam for lam in np.linspace(400e-9, 800e-9, 8)]
+    return w.run(nt, freqs=freqs, dft_stride=8, flags=flags)

@@ -967,6 +967,15 @@ size_t cfd_device_bytes(const cfd_sim *s) { return (s && s->impl) ? s->impl->dev
 
 int cfd_step(cfd_sim *s, int count) { NEED_SOLO(s); return guard([&] { s->impl->step(count); }); }
 int cfd_steps_done(cfd_sim *s) { NEED(s); return s->impl->done; }
+int cfd_query(cfd_sim *s)
+{
+    NEED(s);
+    const hipError_t e = hipStreamQuery(s->impl->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    cfd::g_error = std::string("HIP: ") + hipGetErrorString(e);
+    return -1;
+}
 int cfd_sync(cfd_sim *s)
 {
     NEED(s);

@@ -1,0 +1,354 @@
+// server.cpp -- serving loop over the C ABI: requests (one simulation each)
+// are admitted first-come-first-served while their device memory fits a
+// budget, then stepped in quanta on their own HIP streams, so the kernels of
+// different requests overlap on the GPU (continuous batching across shapes).
+// A finished request's port results are copied to host memory and its device
+// memory freed at once.  One scheduler thread; submit / poll / result are
+// safe from any thread.
+//
+// The reference runs one simulation per process (main.cpp:115-343); this has
+// no counterpart there.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cpsfdtd.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// A deep copy of a cfd_config (the caller's strings and arrays may go away).
+struct ConfigCopy {
+    std::string material, structure, pml, tfsf, dl, gain, source;
+    bool has[7] = {};
+    std::vector<std::string> ports;
+    std::vector<const char *> port_ptrs;
+    std::vector<int> kinds, src_ijkc;
+    std::vector<double> amp, phase, freqs;
+    cfd_config cfg{};
+
+    explicit ConfigCopy(const cfd_config &c) : cfg(c) {
+        const char *in[7] = {c.material, c.structure, c.pml, c.tfsf, c.dl, c.gain, c.source};
+        std::string *out[7] = {&material, &structure, &pml, &tfsf, &dl, &gain, &source};
+        for (int i = 0; i < 7; i++)
+            if (in[i]) { *out[i] = in[i]; has[i] = true; }
+        for (int q = 0; q < c.n_ports; q++) {
+            ports.push_back(c.port_files[q]);
+            kinds.push_back(c.port_kinds[q]);
+        }
+        if (c.n_src > 0) {
+            src_ijkc.assign(c.src_ijkc, c.src_ijkc + 4 * c.n_src);
+            amp.assign(c.src_amp, c.src_amp + c.n_src);
+            phase.assign(c.src_phase, c.src_phase + c.n_src);
+        }
+        if (c.n_freq > 0) freqs.assign(c.freqs, c.freqs + c.n_freq);
+    }
+    const cfd_config *get() {
+        const std::string *s[7] = {&material, &structure, &pml, &tfsf, &dl, &gain, &source};
+        const char **f[7] = {&cfg.material, &cfg.structure, &cfg.pml, &cfg.tfsf, &cfg.dl, &cfg.gain, &cfg.source};
+        for (int i = 0; i < 7; i++) *f[i] = has[i] ? s[i]->c_str() : nullptr;
+        port_ptrs.clear();
+        for (const std::string &p : ports) port_ptrs.push_back(p.c_str());
+        cfg.port_files = port_ptrs.empty() ? nullptr : port_ptrs.data();
+        cfg.port_kinds = kinds.empty() ? nullptr : kinds.data();
+        cfg.src_ijkc = src_ijkc.empty() ? nullptr : src_ijkc.data();
+        cfg.src_amp = amp.empty() ? nullptr : amp.data();
+        cfg.src_phase = phase.empty() ? nullptr : phase.data();
+        cfg.freqs = freqs.empty() ? nullptr : freqs.data();
+        return &cfg;
+    }
+};
+
+struct Request {
+    long long id = 0;
+    int state = CFD_REQ_QUEUED;
+    std::unique_ptr<ConfigCopy> cfg;
+    cfd_sim *sim = nullptr;
+    size_t bytes = 0;
+    int nt = 0, done = 0;
+    bool inflight = false;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    std::string error;
+    std::vector<std::vector<double>> intensity, dft;  // per port
+};
+
+}  // namespace
+
+struct cfd_server {
+    cfd_server_opts opts{};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<long long> queue;
+    std::map<long long, std::unique_ptr<Request>> reqs;
+    std::vector<Request *> active;
+    size_t active_bytes = 0;
+    long long next_id = 1;
+    bool stop = false;
+    Clock::time_point t0 = Clock::now();
+    std::thread worker;
+    std::atomic<long long> steps_launched{0};
+    size_t peak_bytes = 0;
+    int peak_active = 0;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+    // Device bytes a request will need: fields, ID map and CPML psi by cell
+    // count, before its model is built (admission estimate; the real figure
+    // replaces it once the simulation exists).
+    static size_t estimate(const cfd_config &c, const int *n) {
+        const size_t S = c.precision == CFD_F32 ? 4 : 8;
+        const size_t px = ((size_t)n[0] + 1 + 31) / 32 * 32;
+        const size_t cells = px * (n[1] + 1) * (n[2] + 1);
+        return cells * (6 * S + 1 + (c.dl || c.gain ? 4 : 0)) + cells * S;  // + psi and tables, generously
+    }
+
+    void finish(Request &r, bool ok, const std::string &err) {
+        if (ok) {
+            int counts[4] = {0, 0, 0, 0};
+            cfd_info(r.sim, nullptr, nullptr, counts);
+            for (int q = 0; q < counts[0]; q++) {
+                const int npts = cfd_port_npts(r.sim, q);
+                std::vector<double> in((size_t)r.nt), df;
+                if (cfd_port_intensity(r.sim, q, in.data()) < 0) { ok = false; break; }
+                const int nf = r.cfg->cfg.n_freq;
+                if (nf > 0 && npts > 0) {
+                    df.resize((size_t)npts * nf * 12);
+                    if (cfd_port_dft(r.sim, q, df.data()) < 0) { ok = false; break; }
+                }
+                r.intensity.push_back(std::move(in));
+                r.dft.push_back(std::move(df));
+            }
+        }
+        const std::string msg = ok ? std::string() : (err.empty() ? std::string(cfd_last_error()) : err);
+        if (r.sim) cfd_destroy(r.sim);
+        r.sim = nullptr;
+        std::lock_guard<std::mutex> lk(mu);
+        active_bytes -= r.bytes;
+        r.state = ok ? CFD_REQ_DONE : CFD_REQ_FAILED;
+        r.error = msg;
+        r.t_done = now();
+        r.cfg.reset();
+        cv.notify_all();
+    }
+
+    // Admit queued requests, in order, while they fit the budget.
+    void admit() {
+        for (;;) {
+            Request *r = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (queue.empty() || (int)active.size() >= opts.max_active) return;
+                r = reqs[queue.front()].get();
+            }
+            cfd_model *m = nullptr;
+            int n[4] = {0, 0, 0, 0};
+            if (cfd_model_build(r->cfg->get(), &m) < 0) {
+                const std::string err = cfd_model_last_error();
+                { std::lock_guard<std::mutex> lk(mu); queue.pop_front(); }
+                finish(*r, false, err);
+                continue;
+            }
+            cfd_model_dims(m, n, nullptr);
+            cfd_model_free(m);
+            const size_t est = estimate(r->cfg->cfg, n);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!active.empty() && active_bytes + est > opts.hbm_budget) return;  // wait for room (FIFO)
+                queue.pop_front();
+                r->state = CFD_REQ_RUNNING;
+                r->t_start = now();
+            }
+            cfd_sim *s = nullptr;
+            if (cfd_create(r->cfg->get(), &s) < 0) {
+                finish(*r, false, cfd_last_error());
+                continue;
+            }
+            r->sim = s;
+            r->bytes = cfd_device_bytes(s);
+            int info[4];
+            cfd_info(s, info, nullptr, nullptr);
+            r->nt = info[3];
+            std::lock_guard<std::mutex> lk(mu);
+            active.push_back(r);
+            active_bytes += r->bytes;
+            peak_bytes = std::max(peak_bytes, active_bytes);
+            peak_active = std::max(peak_active, (int)active.size());
+        }
+    }
+
+    void run() {
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+                if (stop) break;
+            }
+            admit();
+            // every idle request gets its next quantum (asynchronous, own stream)
+            for (Request *r : std::vector<Request *>(active)) {
+                if (r->inflight) continue;
+                const int k = std::min(opts.quantum, r->nt - r->done);
+                if (k <= 0 || cfd_step(r->sim, k) < 0) {
+                    retire(r, k > 0 ? std::string(cfd_last_error()) : std::string());
+                    continue;
+                }
+                r->done += k;
+                r->inflight = true;
+                steps_launched += k;
+            }
+            // collect finished quanta
+            bool any = false;
+            for (Request *r : std::vector<Request *>(active)) {
+                const int q = cfd_query(r->sim);
+                if (q == 0) continue;
+                any = true;
+                r->inflight = false;
+                if (q < 0) retire(r, cfd_last_error());
+                else if (r->done >= r->nt) retire(r, std::string());
+            }
+            if (!any) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+        for (Request *r : active) finish(*r, false, "server stopped");
+        active.clear();
+    }
+    void retire(Request *r, const std::string &err) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            active.erase(std::find(active.begin(), active.end(), r));
+        }
+        finish(*r, err.empty(), err);
+    }
+};
+
+static thread_local std::string g_server_error;
+
+extern "C" {
+
+int cfd_server_create(const cfd_server_opts *o, cfd_server **out)
+{
+    if (!out) { g_server_error = "null argument"; return -1; }
+    auto s = std::make_unique<cfd_server>();
+    s->opts.hbm_budget = o && o->hbm_budget ? o->hbm_budget : (size_t)200 << 30;
+    s->opts.max_active = o && o->max_active > 0 ? o->max_active : 64;
+    s->opts.quantum = o && o->quantum > 0 ? o->quantum : 100;
+    cfd_server *p = s.get();
+    p->worker = std::thread([p] { p->run(); });
+    *out = s.release();
+    return 0;
+}
+
+int cfd_server_destroy(cfd_server *s)
+{
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    delete s;
+    return 0;
+}
+
+int cfd_server_submit(cfd_server *s, const cfd_config *cfg, long long *id)
+{
+    if (!s || !cfg || !id) { g_server_error = "null argument"; return -1; }
+    auto r = std::make_unique<Request>();
+    r->cfg = std::make_unique<ConfigCopy>(*cfg);
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = s->next_id++;
+    r->t_submit = s->now();
+    *id = r->id;
+    s->queue.push_back(r->id);
+    s->reqs[r->id] = std::move(r);
+    s->cv.notify_all();
+    return 0;
+}
+
+int cfd_server_poll(cfd_server *s, long long id, double *times)
+{
+    if (!s) { g_server_error = "null server"; return -1; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { g_server_error = "no such request"; return -1; }
+    const Request &r = *it->second;
+    if (times) { times[0] = r.t_submit; times[1] = r.t_start; times[2] = r.t_done; }
+    return r.state;
+}
+
+int cfd_server_wait(cfd_server *s, long long id, double timeout_s)
+{
+    if (!s) { g_server_error = "null server"; return -1; }
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { g_server_error = "no such request"; return -1; }
+    Request *r = it->second.get();
+    s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                   [&] { return r->state == CFD_REQ_DONE || r->state == CFD_REQ_FAILED; });
+    return r->state;
+}
+
+int cfd_server_result(cfd_server *s, long long id, int port, int what, double *host, size_t *count)
+{
+    if (!s) { g_server_error = "null server"; return -1; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { g_server_error = "no such request"; return -1; }
+    const Request &r = *it->second;
+    if (r.state == CFD_REQ_FAILED) { g_server_error = "request failed: " + r.error; return -1; }
+    if (r.state != CFD_REQ_DONE) { g_server_error = "request not finished"; return -1; }
+    if (port < 0 || port >= (int)r.intensity.size()) { g_server_error = "no such port"; return -1; }
+    const std::vector<double> &v = what == CFD_RESULT_DFT ? r.dft[port] : r.intensity[port];
+    if (count) *count = v.size();
+    if (host && !v.empty()) std::memcpy(host, v.data(), v.size() * sizeof(double));
+    return 0;
+}
+
+const char *cfd_server_request_error(cfd_server *s, long long id)
+{
+    if (!s) return "null server";
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return "no such request";
+    g_server_error = it->second->error;
+    return g_server_error.c_str();
+}
+
+int cfd_server_release(cfd_server *s, long long id)
+{
+    if (!s) { g_server_error = "null server"; return -1; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { g_server_error = "no such request"; return -1; }
+    if (it->second->state != CFD_REQ_DONE && it->second->state != CFD_REQ_FAILED) {
+        g_server_error = "request not finished";
+        return -1;
+    }
+    s->reqs.erase(it);
+    return 0;
+}
+
+int cfd_server_stats(cfd_server *s, long long *steps_launched, size_t *peak_bytes, int *peak_active, int *queued)
+{
+    if (!s) { g_server_error = "null server"; return -1; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (steps_launched) *steps_launched = s->steps_launched.load();
+    if (peak_bytes) *peak_bytes = s->peak_bytes;
+    if (peak_active) *peak_active = s->peak_active;
+    if (queued) *queued = (int)s->queue.size();
+    return 0;
+}
+
+const char *cfd_server_last_error(void) { return g_server_error.c_str(); }
+
+}  // extern "C"

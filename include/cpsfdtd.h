@@ -78,6 +78,7 @@ size_t cfd_device_bytes(const cfd_sim *s);
 int cfd_step(cfd_sim *s, int count);
 int cfd_steps_done(cfd_sim *s);
 int cfd_sync(cfd_sim *s);
+int cfd_query(cfd_sim *s);               /* 1 = all queued work finished, 0 = still running (no wait) */
 
 /* Single-module passes in the reference's granularity, for drivers that keep
  * main.cpp's call sequence.  n is the time index the reference passes. */
@@ -127,6 +128,37 @@ int cfd_batch_member(cfd_batch *b, int i, cfd_sim **out);
 int cfd_batch_device_bytes(cfd_batch *b, size_t *bytes);
 int cfd_batch_set_timing(cfd_batch *b, int on);
 int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *launches, int cap);
+
+/* Serving: requests (one cfd_config each) admitted first-come-first-served
+ * while their device memory fits hbm_budget, then stepped in quanta of
+ * `quantum` steps on their own streams so that requests of different shapes
+ * run concurrently; a finished request's port results (|E|^2 series, DFT)
+ * are copied to host memory and its device memory freed.  A scheduler thread
+ * runs inside the server; the calls below are thread-safe.  No reference
+ * counterpart (the reference runs one simulation per process,
+ * main.cpp:115-343). */
+#define CFD_REQ_QUEUED  0
+#define CFD_REQ_RUNNING 1
+#define CFD_REQ_DONE    2
+#define CFD_REQ_FAILED  3
+#define CFD_RESULT_INTENSITY 0   /* [steps] sum over the port of |E|^2 (Ports.cpp:512-522) */
+#define CFD_RESULT_DFT       1   /* [points][n_freq][6][re,im] */
+typedef struct {
+    size_t hbm_budget;   /* bytes of device memory for running requests (0: 200 GiB) */
+    int max_active;      /* requests running at once (0: 64) */
+    int quantum;         /* steps per scheduling slice (0: 100) */
+} cfd_server_opts;
+typedef struct cfd_server cfd_server;
+int cfd_server_create(const cfd_server_opts *opts, cfd_server **out);
+int cfd_server_destroy(cfd_server *s);                       /* stops; unfinished requests fail */
+int cfd_server_submit(cfd_server *s, const cfd_config *cfg, long long *id);   /* copies cfg */
+int cfd_server_poll(cfd_server *s, long long id, double *times);  /* returns CFD_REQ_*; times: submit, start, done (s) */
+int cfd_server_wait(cfd_server *s, long long id, double timeout_s);  /* returns CFD_REQ_* */
+int cfd_server_result(cfd_server *s, long long id, int port, int what, double *host, size_t *count);
+const char *cfd_server_request_error(cfd_server *s, long long id);
+int cfd_server_release(cfd_server *s, long long id);         /* drop a finished request's results */
+int cfd_server_stats(cfd_server *s, long long *steps_launched, size_t *peak_bytes, int *peak_active, int *queued);
+const char *cfd_server_last_error(void);
 
 /* Host-side set-up only, no device: the parsed model the engine uploads
  * (Structure, CPML, TF_SF, port constructors).  Usable without a GPU. */

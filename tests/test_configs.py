@@ -65,3 +65,29 @@ def test_c3_model_matches_oracle(tmp_path, oracle_lib):
     assert len(gain) == (idm[:192, :192, 1:64] == 5).sum() > 500000
     assert (idm == 1).sum() > 200000  # the rods
     assert len(m.port_points(0)) > 10000
+
+
+def test_c5_request_stream_statistics():
+    """The serving stream's documented moments: 2000 requests at 20/s, edges
+    log-uniform over 32-320 (mean edge ~125, mean cells ~1.9e6), steps
+    uniform over 500-8000, a 40/30/30 mix."""
+    rq = configs.c5_requests()
+    assert len(rq) == 2000 and rq == configs.c5_requests()
+    t = np.array([r[0] for r in rq])
+    assert np.all(np.diff(t) > 0) and 90 < t[-1] < 110
+    N = np.array([r[1]["N"] for r in rq], dtype=float)
+    assert N.min() >= 32 and N.max() <= 320 and 115 < N.mean() < 135
+    assert 1.6e6 < N.prod(axis=1).mean() < 2.3e6
+    steps = np.array([r[1]["steps"] for r in rq])
+    assert steps.min() >= 500 and steps.max() <= 8000 and 4000 < steps.mean() < 4500
+    kinds = [r[1]["kind"] for r in rq]
+    assert abs(kinds.count("sphere") / 2000 - 0.4) < 0.04 and abs(kinds.count("dl") / 2000 - 0.3) < 0.04
+
+
+def test_c5_request_model_builds(tmp_path):
+    rq = configs.c5_requests(20)
+    for k, (_, p) in enumerate(rq[:6]):
+        d = tmp_path / ("r%d" % k)
+        d.mkdir()
+        m = eng.Model(configs.c5(str(d), p))
+        assert tuple(m.N) == p["N"] and len(m.port_points(0)) > 0

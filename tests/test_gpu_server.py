@@ -1,0 +1,78 @@
+"""The serving loop (cfd_server_*, cpsfdtd_amd.Server): requests of different
+shapes run concurrently on their own streams, in step quanta; each request's
+results must equal a solo run of the same case bit for bit (FP64 and FP32),
+a failing request must not disturb the others, and the memory budget must
+hold requests back."""
+import os
+
+import numpy as np
+import pytest
+
+import cpsfdtd_amd as eng
+from cpsfdtd_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [
+    ({"N": (36, 40, 44), "steps": 150, "kind": "sphere"}, 64),
+    ({"N": (48, 33, 40), "steps": 90, "kind": "dl"}, 64),
+    ({"N": (40, 40, 52), "steps": 120, "kind": "gain"}, 64),
+    ({"N": (33, 45, 38), "steps": 200, "kind": "sphere"}, 32),
+    ({"N": (44, 36, 48), "steps": 75, "kind": "dl"}, 32),
+]
+
+
+def make(root, k, params):
+    d = os.path.join(str(root), "r%d" % k)
+    os.makedirs(d)
+    return configs.c5(d, params)
+
+
+def test_server_results_equal_solo_runs(tmp_path):
+    srv = eng.Server(max_active=3, quantum=37)
+    cases, ids = [], []
+    for k, (p, prec) in enumerate(SMALL):
+        case = make(tmp_path, k, p)
+        cases.append((case, prec))
+        ids.append(srv.submit(case, prec))
+    print("server: %d requests submitted" % len(ids), flush=True)
+    for rid in ids:
+        assert srv.wait(rid, 120.0) == eng.REQ_DONE, srv.error(rid)
+    st = srv.stats()
+    assert st["peak_active"] <= 3 and st["steps_launched"] == sum(p["steps"] for p, _ in SMALL)
+    for rid, (case, prec) in zip(ids, cases):
+        e = eng.Engine(case, prec)
+        e.step(case.nt)
+        assert np.array_equal(srv.result(rid, 0, "intensity"), e.port_intensity(0)), rid
+        assert np.array_equal(srv.result(rid, 0, "dft"), e.port_dft(0)), rid
+        e.close()
+        _, (t_sub, t_start, t_done) = srv.poll(rid)
+        assert 0 <= t_sub <= t_start <= t_done
+        srv.release(rid)
+    srv.close()
+
+
+def test_server_isolates_a_failing_request(tmp_path):
+    srv = eng.Server(max_active=4, quantum=50)
+    good = make(tmp_path, 0, SMALL[0][0])
+    bad = make(tmp_path, 1, SMALL[3][0])
+    os.remove(bad.path("structure.txt"))
+    ids = [srv.submit(good, 64), srv.submit(bad, 64), srv.submit(good, 32)]
+    states = [srv.wait(r, 120.0) for r in ids]
+    assert states == [eng.REQ_DONE, eng.REQ_FAILED, eng.REQ_DONE]
+    assert "structure" in srv.error(ids[1])
+    with pytest.raises(eng.EngineError, match="failed"):
+        srv.result(ids[1], 0)
+    assert srv.result(ids[0], 0).shape == (good.nt,)
+    srv.close()
+
+
+def test_server_budget_admits_one_at_a_time(tmp_path):
+    srv = eng.Server(hbm_budget=1, max_active=8, quantum=25)
+    ids = [srv.submit(make(tmp_path, k, p), prec) for k, (p, prec) in enumerate(SMALL[:3])]
+    assert [srv.wait(r, 120.0) for r in ids] == [eng.REQ_DONE] * 3
+    assert srv.stats()["peak_active"] == 1
+    times = [srv.poll(r)[1] for r in ids]
+    for a, b in zip(times, times[1:]):
+        assert b[1] >= a[2] - 1e-3  # the next started after the previous finished
+    srv.close()
