@@ -138,6 +138,12 @@ public:
     PortDesc<R> *port_tab = nullptr;
     int port_blocks = 0;
     static constexpr int kDftBatch = 16;  // sampled steps buffered per DFT pass
+    // replayed step blocks
+    bool use_graphs = true;
+    int *nbase_dev = nullptr;         // block start read by the captured kernels
+    const int *nbase_cur = nullptr;   // nbase_dev while capturing, else null (absolute steps)
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
     int dft_pending = 0, dft_first = 0;
     double *twE = nullptr, *twH = nullptr;
 
@@ -182,6 +188,8 @@ public:
     ~Sim() override { release(); }
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
         for (void *p : allocs) (void)hipFree(p);
         allocs.clear();
         if (stream && own_stream) (void)hipStreamDestroy(stream);
@@ -353,6 +361,8 @@ public:
         sig_add(port_sig, &m.record, sizeof m.record);
         if (gather_max) gather_buf = alloc<R>((size_t)gather_max * 6);
         setup_passes();
+        nbase_dev = alloc<int>(1);
+        if (const char *v = std::getenv("CFD_GRAPHS")) use_graphs = std::string(v) != "0";
         if (!dports.empty()) {
             port_tab = alloc<PortDesc<R>>(dports.size());
             HIP_OK(hipMemcpyAsync(port_tab, dports.data(), dports.size() * sizeof(PortDesc<R>), hipMemcpyHostToDevice, stream));
@@ -553,16 +563,17 @@ public:
         if (!nt) return;
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
             hipLaunchKernelGGL((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, h ? tfH : tfE, nt,
-                               tcoef, tdelay, tprof, h ? einc : hinc, tablen, n);
+                               tcoef, tdelay, tprof, h ? einc : hinc, tablen, nbase_cur, n);
         });
         HIP_OK(hipGetLastError());
     }
     void sources(int n) override {
         const int ns = (int)(m.src.size() / 4);
         if (!ns) return;
-        if (n < 0 || n > m.nt) throw Error("source time index out of range");
+        if (!nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
         timer.run(stream, "sources", [&] {
-            hipLaunchKernelGGL(k_sources<R>, dim3((ns + 63) / 64), dim3(64), 0, stream, d, scell, scomp, swave, ns, m.nt + 1, n);
+            hipLaunchKernelGGL(k_sources<R>, dim3((ns + 63) / 64), dim3(64), 0, stream, d, scell, scomp, swave, ns, m.nt + 1,
+                               nbase_cur, n);
         });
         HIP_OK(hipGetLastError());
     }
@@ -575,7 +586,8 @@ public:
             slot = dft_pending++;
         }
         timer.run(stream, "ports", [&] {
-            hipLaunchKernelGGL((k_port<R>), dim3(port_blocks), dim3(256), 0, stream, d, nullptr, port_tab, (int)dports.size(), n, slot);
+            hipLaunchKernelGGL((k_port<R>), dim3(port_blocks), dim3(256), 0, stream, d, nullptr, port_tab, (int)dports.size(),
+                               nbase_cur, n, slot);
         });
         HIP_OK(hipGetLastError());
         if (dft_pending == kDftBatch) dft_flush();
@@ -586,7 +598,7 @@ public:
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
             hipLaunchKernelGGL((k_dft<R>), dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
-                               dft_first, m.dft_stride, twE, twH, (int)m.freqs.size(), m.nt);
+                               nbase_cur, dft_first, m.dft_stride, twE, twH, (int)m.freqs.size(), m.nt);
         });
         HIP_OK(hipGetLastError());
     }
@@ -599,17 +611,68 @@ public:
     // main.cpp:267-343: one fused H pass (basic + CPML), TF/SF-H, one fused E
     // pass (gain polarisation, basic, Drude-Lorentz, gain current and
     // populations, CPML), TF/SF-E, sources, ports
+    void one_step(int n) {
+        launch_update(true);
+        tfsf(true, n);
+        launch_update(false);
+        tfsf(false, n);
+        sources(n);
+        if (!dports.empty()) ports(n);
+    }
+
+    // Step blocks replayed from a HIP graph: a block of graph_block() steps
+    // starting at a multiple of its length is captured once, its
+    // step-dependent kernels reading the block start from *nbase_dev and
+    // their offset from the captured arguments.  The block length is a
+    // multiple of dft_stride x kDftBatch, so the DFT staging pattern (slots,
+    // flushes) is the same in every block.  The kernels and arguments are
+    // those of the eager path, so replays are bit-identical to it.
+    int graph_block() const {
+        if (!use_graphs) return 0;
+        if (m.freqs.empty()) return 128;
+        const int unit = m.dft_stride * kDftBatch;
+        return unit >= 128 ? unit : unit * (128 / unit);
+    }
+    void capture_block(int G) {
+        const bool timing = timer.on;
+        timer.on = false;
+        nbase_cur = nbase_dev;
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i < G; i++) one_step(i);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            nbase_cur = nullptr;
+            timer.on = timing;
+            throw;
+        }
+        nbase_cur = nullptr;
+        timer.on = timing;
+        HIP_OK(hipStreamEndCapture(stream, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_OK(e);
+        graph_steps = G;
+    }
     void step(int count) override {
-        for (int c = 0; c < count; c++) {
+        while (count > 0) {
             const int n = done;
             if (n >= m.nt) throw Error("run finished: all " + std::to_string(m.nt) + " steps done");
-            launch_update(true);
-            tfsf(true, n);
-            launch_update(false);
-            tfsf(false, n);
-            sources(n);
-            if (!dports.empty()) ports(n);
+            const int G = graph_block();
+            if (G && count >= G && n % G == 0 && n + G <= m.nt && dft_pending == 0 && !timer.on) {
+                if (!gexec) capture_block(G);
+                hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, stream, nbase_dev, n);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipGraphLaunch(gexec, stream));
+                done += G;
+                count -= G;
+                continue;
+            }
+            one_step(n);
             done++;
+            count--;
         }
     }
 
@@ -827,7 +890,7 @@ public:
         if (!nt) return;
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
             hipLaunchKernelGGL((k_tfsf<R, true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
-                               h ? a.tfH : a.tfE, nt, a.tcoef, a.tdelay, a.tprof, h ? a.einc : a.hinc, a.tablen, n);
+                               h ? a.tfH : a.tfE, nt, a.tcoef, a.tdelay, a.tprof, h ? a.einc : a.hinc, a.tablen, nullptr, n);
         });
         HIP_OK(hipGetLastError());
     }
@@ -841,7 +904,7 @@ public:
         }
         timer.run(stream, "ports", [&] {
             hipLaunchKernelGGL((k_port<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
-                               ports_all, nports, n, slot);
+                               ports_all, nports, nullptr, n, slot);
         });
         HIP_OK(hipGetLastError());
         if (dft_pending == Sim<R>::kDftBatch) dft_flush();
@@ -853,7 +916,7 @@ public:
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
             hipLaunchKernelGGL((k_dft<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, ports_all,
-                               nports, nbatch, dft_first, a.m.dft_stride, a.twE, a.twH, (int)a.m.freqs.size(), a.m.nt);
+                               nports, nbatch, nullptr, dft_first, a.m.dft_stride, a.twE, a.twH, (int)a.m.freqs.size(), a.m.nt);
         });
         HIP_OK(hipGetLastError());
     }

@@ -671,9 +671,10 @@ struct TfTarget {
 template <class R, bool B = false>
 __global__ __launch_bounds__(256) void k_tfsf(Dev<R> d0, const Dev<R> *devs, const TfTarget *tg, int ntg,
                                               const double *coef, const double *delay, const double *prof,
-                                              const double *tab, int tablen, int n)
+                                              const double *tab, int tablen, const int *nbase, int n)
 {
     const Dev<R> &d = instance<R, B>(d0, devs);
+    if (nbase) n += *nbase;  // replayed step blocks: step = block start + offset
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= ntg) return;
     const TfTarget g = tg[t];
@@ -697,10 +698,18 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, const long long *cell,
     for (int c = 0; c < 6; c++) out[(long long)t * 6 + c] = d.F[c][p];
 }
 
+// the block start of a replayed step graph
+__global__ void k_set_step(int *nbase, int n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *nbase = n;
+}
+
 // soft point sources (main.cpp:284-288)
 template <class R>
-__global__ void k_sources(Dev<R> d, const long long *cell, const int *comp, const double *wave, int nsrc, int stride, int n)
+__global__ void k_sources(Dev<R> d, const long long *cell, const int *comp, const double *wave, int nsrc, int stride,
+                          const int *nbase, int n)
 {
+    if (nbase) n += *nbase;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nsrc) return;
     R *F = d.F[comp[q]];
@@ -724,8 +733,9 @@ struct PortDesc {
 // the workgroup with wavefront shuffles into one partial per workgroup.
 template <class R, bool B = false>
 __global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, const PortDesc<R> *ports, int nports,
-                                              int n, int slot)
+                                              const int *nbase, int n, int slot)
 {
+    if (nbase) n += *nbase;
     const Dev<R> &d = instance<R, B>(d0, devs);
     if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     __shared__ double red[4];
@@ -757,9 +767,10 @@ __global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, con
 // additions, in the same order, as accumulating every sampled step), and is
 // written once.  E samples carry phase 2 pi f (n+1) dt, H samples (n+1/2) dt.
 template <class R, bool B = false>
-__global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nports, int nb, int n0, int stride,
-                                             const double *twE, const double *twH, int nf, int ntot)
+__global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nports, int nb, const int *nbase, int n0,
+                                             int stride, const double *twE, const double *twH, int nf, int ntot)
 {
+    if (nbase) n0 += *nbase;
     if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     int q = 0;
     while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
