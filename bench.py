@@ -221,6 +221,53 @@ def batched_c4(steps=200, warmup=10, n=512):
             "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()}}
 
 
+def serving_c5(nreq=120, precision=32, quantum=256):
+    """The first ``nreq`` requests of the serving workload's stream (Poisson
+    20/s, log-uniform 32-320 edges, 500-8000 steps, sphere / Drude-Lorentz /
+    gain mix), submitted at their arrival times to the server: latency from
+    arrival to port results on the host, p50 and p99."""
+    import numpy as np
+    import cpsfdtd_amd as eng
+    from cpsfdtd_amd import configs
+    root = tempfile.mkdtemp(prefix="bench_c5_")
+    rq = configs.c5_requests(nreq)
+    cases = []
+    for k, (t, p) in enumerate(rq):
+        d = os.path.join(root, "r%d" % k)
+        os.makedirs(d)
+        cases.append((t, configs.c5(d, p)))
+    work = sum(p["N"][0] * p["N"][1] * p["N"][2] * p["steps"] for _, p in rq)
+    srv = eng.Server(quantum=quantum)
+    t0 = time.perf_counter()
+    ids = []
+    for t, case in cases:
+        dt = t0 + t - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(case, precision))
+    lat, first, last = [], None, 0.0
+    for rid in ids:
+        st = srv.wait(rid, 900.0)
+        if st != eng.REQ_DONE:
+            raise RuntimeError("request %d: %s" % (rid, srv.error(rid)))
+        _, (ts, tb, td) = srv.poll(rid)
+        lat.append(td - ts)
+        first = ts if first is None else min(first, ts)
+        last = max(last, td)
+        srv.release(rid)
+    stats = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    span = last - first
+    return {"workload": "c5: first %d requests of the seeded stream (Poisson 20/s), FP%d" % (nreq, precision),
+            "requests": nreq, "p50_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_s": round(float(np.percentile(lat, 99)), 3), "mean_s": round(float(lat.mean()), 3),
+            "value": round(work / span / 1e6, 2), "unit": "Mcell-updates/s (served)",
+            "arrival_span_s": round(rq[-1][0], 2), "makespan_s": round(span, 2),
+            "demand_mcells": round(work / rq[-1][0] / 1e6, 1), "peak_active": stats["peak_active"],
+            "peak_device_gb": round(stats["peak_bytes"] / 1e9, 2), "quantum": quantum}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +277,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched c4 measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served c5 measurement")
+    ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -334,6 +383,10 @@ def main():
         log("bench: batched c4 (512 unit cells)")
         out["batched"] = batched_c4()
         log("bench: batched c4 %.1f Mcell-upd/s" % out["batched"]["value"])
+    if not args.no_serve:
+        log("bench: serving the first %d c5 requests" % args.serve_requests)
+        out["serving"] = serving_c5(args.serve_requests)
+        log("bench: serving p50 %.2f s, p99 %.2f s" % (out["serving"]["p50_s"], out["serving"]["p99_s"]))
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()

@@ -75,6 +75,7 @@ SIGNATURES = {
     "cfd_port_intensity": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_gain_read": ([_vp, _dp], C.c_int),
     "cfd_model_build": ([C.POINTER(_Config), C.POINTER(_vp)], C.c_int),
+    "cfd_create_from_model": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
     "cfd_model_free": ([_vp], C.c_int),
     "cfd_model_last_error": ([], C.c_char_p),
     "cfd_model_dims": ([_vp, _ip, _dp], C.c_int),

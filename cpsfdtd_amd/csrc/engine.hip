@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,15 @@ struct KernelTimer {
 };
 
 // ----------------------------------------------------------------------------
+// Held while a step block is captured into a graph and while a simulation
+// allocates or releases device memory: stream-ordered allocations are refused
+// while any stream of the process is capturing, and simulations are created
+// and released on other threads (a serving loop) while others capture.
+inline std::mutex &capture_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 // FNV-1a over raw bytes: a batch checks that its instances share a geometry
 inline void sig_add(uint64_t &h, const void *p, size_t n) {
     const unsigned char *b = static_cast<const unsigned char *>(p);
@@ -147,11 +157,26 @@ public:
     int dft_pending = 0, dft_first = 0;
     double *twE = nullptr, *twH = nullptr;
 
+    // Stream-ordered allocations from the device's default pool, which keeps
+    // freed memory for reuse: creating and releasing simulations (a serving
+    // loop does it all the time) never synchronises the whole device.
+    static void keep_pool_memory() {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            int dev = 0;
+            hipMemPool_t pool;
+            if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t keep = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+        });
+    }
     template <class T>
     T *alloc(size_t count) {
         if (!count) count = 1;
         void *p = nullptr;
-        HIP_OK(hipMalloc(&p, count * sizeof(T)));
+        keep_pool_memory();
+        HIP_OK(hipMallocAsync(&p, count * sizeof(T), stream));
         HIP_OK(hipMemsetAsync(p, 0, count * sizeof(T), stream));
         allocs.push_back(p);
         dev_bytes += count * sizeof(T);
@@ -179,6 +204,7 @@ public:
             HIP_OK(hipStreamCreate(&stream));
         }
         try {
+            std::lock_guard<std::mutex> lk(capture_mutex());
             setup();
         } catch (...) {
             release();
@@ -188,10 +214,12 @@ public:
     ~Sim() override { release(); }
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
+        std::lock_guard<std::mutex> lk(capture_mutex());
         if (gexec) (void)hipGraphExecDestroy(gexec);
         gexec = nullptr;
-        for (void *p : allocs) (void)hipFree(p);
+        for (void *p : allocs) (void)hipFreeAsync(p, stream);
         allocs.clear();
+        if (stream) (void)hipStreamSynchronize(stream);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -638,7 +666,8 @@ public:
         timer.on = false;
         nbase_cur = nbase_dev;
         hipGraph_t g = nullptr;
-        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        std::lock_guard<std::mutex> lk(capture_mutex());
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         try {
             for (int i = 0; i < G; i++) one_step(i);
         } catch (...) {
@@ -809,6 +838,7 @@ public:
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
         sims.clear();
+        std::lock_guard<std::mutex> lk(capture_mutex());
         for (void *p : allocs) (void)hipFree(p);
         allocs.clear();
         if (stream) (void)hipStreamDestroy(stream);
@@ -817,6 +847,7 @@ public:
     template <class T>
     T *upload(const std::vector<T> &v) {
         void *p = nullptr;
+        std::lock_guard<std::mutex> lk(capture_mutex());
         HIP_OK(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)));
         allocs.push_back(p);
         own_bytes += v.size() * sizeof(T);
@@ -997,6 +1028,24 @@ int cfd_create(const cfd_config *cfg, cfd_sim **out)
         cfd::Model m = cfd::build_model(*cfg);
         auto s = std::make_unique<cfd_sim>();
         if (cfg->precision == CFD_F32) s->owned = std::make_unique<cfd::Sim<float>>(std::move(m), 32);
+        else s->owned = std::make_unique<cfd::Sim<double>>(std::move(m), 64);
+        s->impl = s->owned.get();
+        *out = s.release();
+    });
+}
+
+int cfd_create_from_model(cfd_model *model, int precision, cfd_sim **out)
+{
+    if (!out) { delete model; cfd::g_error = "null argument"; return -1; }
+    *out = nullptr;
+    if (!model) { cfd::g_error = "null argument"; return -1; }
+    std::unique_ptr<cfd_model> owned(model);
+    return guard([&] {
+        if (precision != CFD_F32 && precision != CFD_F64) throw cfd::Error("precision must be 32 or 64");
+        cfd::Model m = std::move(owned->m);
+        owned.reset();
+        auto s = std::make_unique<cfd_sim>();
+        if (precision == CFD_F32) s->owned = std::make_unique<cfd::Sim<float>>(std::move(m), 32);
         else s->owned = std::make_unique<cfd::Sim<double>>(std::move(m), 64);
         s->impl = s->owned.get();
         *out = s.release();

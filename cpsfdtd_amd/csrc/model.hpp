@@ -103,3 +103,8 @@ struct Model {
 Model build_model(const cfd_config &cfg);
 
 }  // namespace cfd
+
+// the C ABI's opaque model handle (cfd_model_build, cfd_create_from_model)
+struct cfd_model {
+    cfd::Model m;
+};

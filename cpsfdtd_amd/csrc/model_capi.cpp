@@ -6,9 +6,6 @@
 #include "cpsfdtd.h"
 #include "model.hpp"
 
-struct cfd_model {
-    cfd::Model m;
-};
 
 namespace {
 thread_local std::string g_model_error;

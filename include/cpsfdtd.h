@@ -145,8 +145,8 @@ int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *lau
 #define CFD_RESULT_DFT       1   /* [points][n_freq][6][re,im] */
 typedef struct {
     size_t hbm_budget;   /* bytes of device memory for running requests (0: 200 GiB) */
-    int max_active;      /* requests running at once (0: 64) */
-    int quantum;         /* steps per scheduling slice (0: 100) */
+    int max_active;      /* requests running at once (0: 4) */
+    int quantum;         /* steps per scheduling slice (0: 256) */
 } cfd_server_opts;
 typedef struct cfd_server cfd_server;
 int cfd_server_create(const cfd_server_opts *opts, cfd_server **out);
@@ -164,6 +164,8 @@ const char *cfd_server_last_error(void);
  * (Structure, CPML, TF_SF, port constructors).  Usable without a GPU. */
 typedef struct cfd_model cfd_model;
 int cfd_model_build(const cfd_config *cfg, cfd_model **out);
+/* the device simulation of a built model; consumes the model (also on error) */
+int cfd_create_from_model(cfd_model *model, int precision, cfd_sim **out);
 int cfd_model_free(cfd_model *m);
 const char *cfd_model_last_error(void);
 int cfd_model_dims(cfd_model *m, int *nxyz_nt, double *d);
