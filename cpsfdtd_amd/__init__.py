@@ -96,6 +96,7 @@ SIGNATURES = {
     "cfd_batch_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
     "cfd_set_timing": ([_vp, C.c_int], C.c_int),
     "cfd_query": ([_vp], C.c_int),
+    "cfd_check_finite": ([_vp], C.c_int),
     "cfd_server_create": ([C.c_void_p, C.POINTER(_vp)], C.c_int),
     "cfd_server_destroy": ([_vp], C.c_int),
     "cfd_server_submit": ([_vp, C.POINTER(_Config), C.POINTER(C.c_longlong)], C.c_int),
@@ -280,6 +281,10 @@ class Engine:
         a = np.zeros((self.steps_done, n, 6), np.float64)
         _check(library().cfd_port_samples(self.h, q, a.ctypes.data_as(_dp)))
         return a
+
+    def nonfinite(self):
+        """Number of NaN / inf values in the six fields now."""
+        return _check(library().cfd_check_finite(self.h))
 
     def port_gather(self, q):
         """Current field values at port q's points, [points][6]."""

@@ -125,6 +125,7 @@ public:
     virtual void port_dft(int q, double *host) = 0;
     virtual void port_intensity(int q, double *host) = 0;
     virtual void gain_read(double *host) = 0;
+    virtual long long nonfinite() = 0;
 };
 
 template <class R>
@@ -151,6 +152,7 @@ public:
     // replayed step blocks
     bool use_graphs = true;
     int *nbase_dev = nullptr;         // block start read by the captured kernels
+    int *count_dev = nullptr;         // the non-finite watchdog's counter
     const int *nbase_cur = nullptr;   // nbase_dev while capturing, else null (absolute steps)
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
@@ -390,6 +392,7 @@ public:
         if (gather_max) gather_buf = alloc<R>((size_t)gather_max * 6);
         setup_passes();
         nbase_dev = alloc<int>(1);
+        count_dev = alloc<int>(1);
         if (const char *v = std::getenv("CFD_GRAPHS")) use_graphs = std::string(v) != "0";
         if (!dports.empty()) {
             port_tab = alloc<PortDesc<R>>(dports.size());
@@ -692,7 +695,7 @@ public:
             const int G = graph_block();
             if (G && count >= G && n % G == 0 && n + G <= m.nt && dft_pending == 0 && !timer.on) {
                 if (!gexec) capture_block(G);
-                hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, stream, nbase_dev, n);
+                hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, stream, nbase_dev, n);
                 HIP_OK(hipGetLastError());
                 HIP_OK(hipGraphLaunch(gexec, stream));
                 done += G;
@@ -731,6 +734,16 @@ public:
         if (n) HIP_OK(hipMemcpyAsync(tmp.data(), p.samples, n * sizeof(R), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
         for (size_t t = 0; t < n; t++) host[t] = (double)tmp[t];
+    }
+    long long nonfinite() override {
+        const long long n = d.slab * (m.Nz + 1);
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, stream, count_dev, 0);
+        hipLaunchKernelGGL(k_nonfinite<R>, dim3(1024), dim3(256), 0, stream, d, n, count_dev);
+        HIP_OK(hipGetLastError());
+        int h = 0;
+        HIP_OK(hipMemcpyAsync(&h, count_dev, sizeof h, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return h;
     }
     void port_gather(int q, double *host) override {
         const PortDesc<R> &p = dports.at(q);
@@ -1079,6 +1092,13 @@ size_t cfd_device_bytes(const cfd_sim *s) { return (s && s->impl) ? s->impl->dev
 
 int cfd_step(cfd_sim *s, int count) { NEED_SOLO(s); return guard([&] { s->impl->step(count); }); }
 int cfd_steps_done(cfd_sim *s) { NEED(s); return s->impl->done; }
+int cfd_check_finite(cfd_sim *s)
+{
+    NEED(s);
+    long long n = 0;
+    const int rc = guard([&] { n = s->impl->nonfinite(); });
+    return rc ? rc : (int)std::min<long long>(n, 0x7fffffff);
+}
 int cfd_query(cfd_sim *s)
 {
     NEED(s);

@@ -698,10 +698,30 @@ __global__ __launch_bounds__(256) void k_gather(Dev<R> d, const long long *cell,
     for (int c = 0; c < 6; c++) out[(long long)t * 6 + c] = d.F[c][p];
 }
 
-// the block start of a replayed step graph
-__global__ void k_set_step(int *nbase, int n)
+// Non-finite values in the six fields (the serving loop's watchdog): one
+// count per workgroup, added to *count (zeroed by the caller's launch of
+// k_set_int).  Reads every field once.
+template <class R>
+__global__ __launch_bounds__(256) void k_nonfinite(Dev<R> d, long long n, int *count)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *nbase = n;
+    __shared__ int red[4];
+    int bad = 0;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256)
+        for (int c = 0; c < 6; c++) bad += isfinite(d.F[c][q]) ? 0 : 1;
+    for (int off = 32; off > 0; off >>= 1) bad += __shfl_down(bad, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bad;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = red[0] + red[1] + red[2] + red[3];
+        if (t) atomicAdd(count, t);
+    }
+}
+
+// one device int: the block start of a replayed step graph, or a counter's
+// zero (a kernel, not a memset, so it may sit inside a captured graph)
+__global__ void k_set_int(int *dst, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
 }
 
 // soft point sources (main.cpp:284-288)

@@ -252,8 +252,16 @@ struct cfd_server {
                 if (q == 0) continue;
                 any = true;
                 r->inflight = false;
-                if (q < 0) retire(r, cfd_last_error());
-                else if (r->done >= r->nt) retire(r, std::string());
+                if (q < 0) { retire(r, cfd_last_error()); continue; }
+                // watchdog: a request whose fields went non-finite fails now,
+                // alone, instead of running on and returning garbage
+                const int bad = cfd_check_finite(r->sim);
+                if (bad != 0) {
+                    retire(r, bad < 0 ? std::string(cfd_last_error())
+                                      : "non-finite field values (" + std::to_string(bad) + ") by step " + std::to_string(r->done));
+                    continue;
+                }
+                if (r->done >= r->nt) retire(r, std::string());
             }
             if (!any) std::this_thread::sleep_for(std::chrono::microseconds(50));
         }

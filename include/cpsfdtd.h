@@ -78,7 +78,10 @@ size_t cfd_device_bytes(const cfd_sim *s);
 int cfd_step(cfd_sim *s, int count);
 int cfd_steps_done(cfd_sim *s);
 int cfd_sync(cfd_sim *s);
-int cfd_query(cfd_sim *s);               /* 1 = all queued work finished, 0 = still running (no wait) */
+int cfd_query(cfd_sim *s);
+/* number of non-finite (NaN, inf) field values now (0: healthy); reads every
+ * field once.  No reference counterpart (the reference checks nothing). */
+int cfd_check_finite(cfd_sim *s);               /* 1 = all queued work finished, 0 = still running (no wait) */
 
 /* Single-module passes in the reference's granularity, for drivers that keep
  * main.cpp's call sequence.  n is the time index the reference passes. */

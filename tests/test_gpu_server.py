@@ -76,3 +76,36 @@ def test_server_budget_admits_one_at_a_time(tmp_path):
     for a, b in zip(times, times[1:]):
         assert b[1] >= a[2] - 1e-3  # the next started after the previous finished
     srv.close()
+
+
+def test_server_watchdog_fails_a_diverging_request(tmp_path):
+    """A request whose medium has a large negative conductivity (|aE| > 1,
+    growing fields) overflows to inf/NaN; the watchdog fails it with a clear
+    message while a healthy request beside it completes."""
+    good = make(tmp_path, 0, SMALL[0][0])
+    bad = make(tmp_path, 1, {"N": (36, 36, 36), "steps": 400, "kind": "sphere"})
+    with open(bad.path("material.txt")) as fh:
+        lines = fh.read().splitlines()
+    assert lines[2].split()[0] == "2.25"  # the sphere's medium (epsR muR sigma sigmaM)
+    lines[2] = "2.25\t1\t-2e6\t0"
+    with open(bad.path("material.txt"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    srv = eng.Server(max_active=2, quantum=64)
+    ids = [srv.submit(bad, 32), srv.submit(good, 32)]
+    assert srv.wait(ids[0], 120.0) == eng.REQ_FAILED
+    assert "non-finite" in srv.error(ids[0])
+    assert srv.wait(ids[1], 120.0) == eng.REQ_DONE
+    srv.close()
+
+
+def test_engine_nonfinite_count(tmp_path):
+    case = make(tmp_path, 0, SMALL[0][0])
+    e = eng.Engine(case, 64)
+    e.step(10)
+    assert e.nonfinite() == 0
+    f = e.field(1)
+    f[3, 4, 5] = np.nan
+    f[6, 7, 8] = np.inf
+    e.set_field(1, f)
+    assert e.nonfinite() == 2
+    e.close()
