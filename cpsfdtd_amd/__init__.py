@@ -66,6 +66,7 @@ SIGNATURES = {
     "cfd_advance": ([_vp], C.c_int),
     "cfd_read_field": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_write_field": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_read_region": ([_vp, C.c_int, _ip, _dp], C.c_int),
     "cfd_read_id": ([_vp, C.POINTER(C.c_int16)], C.c_int),
     "cfd_port_npts": ([_vp, C.c_int], C.c_int),
     "cfd_port_points": ([_vp, C.c_int, _ip], C.c_int),
@@ -83,7 +84,7 @@ SIGNATURES = {
     "cfd_model_table": ([_vp, C.c_char_p, _dp], C.c_int),
     "cfd_model_tfsf": ([_vp, C.c_int, _ip, _dp, _ip], C.c_int),
     "cfd_model_port": ([_vp, C.c_int, _ip], C.c_int),
-    "cfd_model_cells": ([_vp, C.c_int, _ip], C.c_int),
+    "cfd_model_cells": ([_vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
     "cfd_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
     "cfd_batch_create": ([C.POINTER(_Config), C.c_int, C.POINTER(_vp)], C.c_int),
     "cfd_batch_destroy": ([_vp], C.c_int),
@@ -280,6 +281,13 @@ class Engine:
         n = _check(library().cfd_port_npts(self.h, q))
         a = np.zeros((self.steps_done, n, 6), np.float64)
         _check(library().cfd_port_samples(self.h, q, a.ctypes.data_as(_dp)))
+        return a
+
+    def region(self, comp, i0, i1, j0, j1, k0, k1):
+        """Cells [i0,i1) x [j0,j1) x [k0,k1) of one component, [i][j][k]."""
+        box = np.array([i0, i1, j0, j1, k0, k1], np.int32)
+        a = np.empty((i1 - i0, j1 - j0, k1 - k0), np.float64)
+        _check(library().cfd_read_region(self.h, comp, box.ctypes.data_as(_ip), a.ctypes.data_as(_dp)))
         return a
 
     def nonfinite(self):
@@ -523,6 +531,6 @@ class Model:
     def cells(self, gain=False):
         L = library()
         n = L.cfd_model_cells(self.h, 1 if gain else 0, None)
-        a = np.empty(max(n, 1), np.int32)
-        L.cfd_model_cells(self.h, 1 if gain else 0, a.ctypes.data_as(_ip))
+        a = np.empty(max(n, 1), np.int64)
+        L.cfd_model_cells(self.h, 1 if gain else 0, a.ctypes.data_as(C.POINTER(C.c_longlong)))
         return a[:n]

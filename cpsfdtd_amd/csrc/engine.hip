@@ -120,6 +120,7 @@ public:
     virtual void ports(int n) = 0;
     virtual void read_field(int comp, double *host) = 0;
     virtual void write_field(int comp, const double *host) = 0;
+    virtual void read_region(int comp, const int *box, double *host) = 0;
     virtual void port_samples(int q, double *host) = 0;
     virtual void port_gather(int q, double *host) = 0;
     virtual void port_dft(int q, double *host) = 0;
@@ -275,7 +276,7 @@ public:
         // so the lanes of a wavefront touch consecutive state words
         if (m.has_dl || m.has_gain) {
             std::vector<int> aux(ncell, -1);
-            auto assign = [&](const std::vector<int> &cells, std::vector<int> &slot_of) {
+            auto assign = [&](const std::vector<long long> &cells, std::vector<int> &slot_of) {
                 std::vector<std::pair<long long, int>> order;
                 order.reserve(cells.size());
                 for (size_t r = 0; r < cells.size(); r++) {
@@ -508,9 +509,10 @@ public:
         auto interior = [&](int i, int j, int k) {
             return i >= leanE.i0 && i < leanE.i1 && j >= leanE.j0 && j < leanE.j1 && k >= leanE.k0 && k < leanE.k1;
         };
-        for (const std::vector<int> *cells : {&m.dl_cells, &m.gain_cells})
-            for (int f : *cells) {
-                const int i = f / ((m.Ny + 1) * (m.Nz + 1)), j = (f / (m.Nz + 1)) % (m.Ny + 1), k = f % (m.Nz + 1);
+        for (const std::vector<long long> *cells : {&m.dl_cells, &m.gain_cells})
+            for (const long long f : *cells) {
+                const long long plane = (long long)(m.Ny + 1) * (m.Nz + 1);
+                const int i = (int)(f / plane), j = (int)((f / (m.Nz + 1)) % (m.Ny + 1)), k = (int)(f % (m.Nz + 1));
                 sp.push_back(dev_index(i, j, k) | (interior(i, j, k) ? 0LL : 1LL << kSpecialPml));
             }
         const long long mask = (1LL << kSpecialPml) - 1;
@@ -716,6 +718,22 @@ public:
         for (int i = 0; i <= m.Nx; i++)
             for (int j = 0; j <= m.Ny; j++)
                 for (int k = 0; k <= m.Nz; k++) host[m.flat(i, j, k)] = (double)tmp[dev_index(i, j, k)];
+    }
+    // cells [i0,i1) x [j0,j1) x [k0,k1) of one component, host [i][j][k]
+    void read_region(int comp, const int *b, double *host) override {
+        const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
+        if (ni <= 0 || nj <= 0 || nk <= 0 || b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] > m.Nx + 1 || b[3] > m.Ny + 1 ||
+            b[5] > m.Nz + 1)
+            throw Error("region outside the grid");
+        std::vector<R> row((size_t)ni * nj * nk);
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                HIP_OK(hipMemcpyAsync(row.data() + ((size_t)k * nj + j) * ni, d.F[comp] + dev_index(b[0], b[2] + j, b[4] + k),
+                                      ni * sizeof(R), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (int i = 0; i < ni; i++)
+            for (int j = 0; j < nj; j++)
+                for (int k = 0; k < nk; k++) host[((size_t)i * nj + j) * nk + k] = (double)row[((size_t)k * nj + j) * ni + i];
     }
     void write_field(int comp, const double *host) override {
         const size_t ncell = (size_t)d.slab * (m.Nz + 1);
@@ -1098,6 +1116,12 @@ int cfd_check_finite(cfd_sim *s)
     long long n = 0;
     const int rc = guard([&] { n = s->impl->nonfinite(); });
     return rc ? rc : (int)std::min<long long>(n, 0x7fffffff);
+}
+int cfd_read_region(cfd_sim *s, int comp, const int *box, double *host)
+{
+    NEED(s);
+    if (comp < 0 || comp > 5 || !box || !host) { cfd::g_error = "bad component or null buffer"; return -1; }
+    return guard([&] { s->impl->read_region(comp, box, host); });
 }
 int cfd_query(cfd_sim *s)
 {

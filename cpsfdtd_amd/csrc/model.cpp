@@ -188,7 +188,8 @@ void load_structure(Model &m, const char *matf, const char *meshf) {
     m.Ny = (int)(m.Ly / m.dy);
     m.Nz = (int)(m.Lz / m.dz);
     if (m.Nx < 2 || m.Ny < 2 || m.Nz < 2) throw Error("grid must have at least 2 cells per axis");
-    if ((double)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1) > 2.0e9) throw Error("grid too large for one simulation");
+    // cell indices are 64-bit throughout; beyond 2^36 cells no device memory holds the fields anyway
+    if ((double)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1) > 68719476736.0) throw Error("grid too large for one simulation");
     m.dt = 0.99 / (kC * std::sqrt(1.0 / (m.dx * m.dx) + 1.0 / (m.dy * m.dy) + 1.0 / (m.dz * m.dz)));
     m.id.assign(m.cells(), 0);
     if (!freespace) {
@@ -622,8 +623,8 @@ void cell_lists(Model &m) {
         for (int j = je0; j < m.Ny; j++)
             for (int k = 1; k < m.Nz; k++) {
                 const size_t p = m.flat(i, j, k);
-                if (m.has_dl && m.id[p] == 2) m.dl_cells.push_back((int)p);
-                if (m.has_gain && m.id[p] == 5) m.gain_cells.push_back((int)p);
+                if (m.has_dl && m.id[p] == 2) m.dl_cells.push_back((long long)p);
+                if (m.has_gain && m.id[p] == 5) m.gain_cells.push_back((long long)p);
             }
 }
 

@@ -76,13 +76,13 @@ struct Model {
     int nD = 0, nL = 0;
     double dlC1 = 0, dlC2 = 0, dlC3 = 0, dl_epsInf = 0;
     std::vector<double> dlC4, dlC5, dlC6, dlKap, dlBeta, dlAlpha, dlXi, dlGam;
-    std::vector<int> dl_cells;  // flat [i][j][k] index
+    std::vector<long long> dl_cells;  // flat [i][j][k] index
 
     // four-level gain (cells with ID == 5)
     bool has_gain = false;
     double g_lambda = 0, g_gamma = 0, g_t32 = 0, g_t21 = 0, g_t10 = 0, g_Ntot = 0, g_Wp = 0;
     double g_omega = 0, g_kappa = 0, gCPa1 = 0, gCPa2 = 0, gCPa3 = 0;
-    std::vector<int> gain_cells;
+    std::vector<long long> gain_cells;
 
     // point sources
     std::vector<int> src;          // i j k comp

@@ -117,11 +117,11 @@ int cfd_model_port(cfd_model *p, int port, int *ijk)
     return (int)(v.size() / 3);
 }
 
-int cfd_model_cells(cfd_model *p, int gain, int *flat)
+int cfd_model_cells(cfd_model *p, int gain, long long *flat)
 {
     if (!p) return -1;
     const auto &v = gain ? p->m.gain_cells : p->m.dl_cells;
-    if (flat) std::memcpy(flat, v.data(), v.size() * sizeof(int));
+    if (flat) std::memcpy(flat, v.data(), v.size() * sizeof(long long));
     return (int)v.size();
 }
 

@@ -102,6 +102,10 @@ int cfd_advance(cfd_sim *s);            /* count one step done after a module se
 /* Host access (synchronising). */
 int cfd_read_field(cfd_sim *s, int comp, double *host);        /* comp 0..5 Ex Ey Ez Hx Hy Hz */
 int cfd_write_field(cfd_sim *s, int comp, const double *host);
+/* cells [box0,box1) x [box2,box3) x [box4,box5) of one component, host
+ * [i][j][k] k fastest: a sub-volume of Field::Ex...Hz without copying the
+ * whole array (large grids) */
+int cfd_read_region(cfd_sim *s, int comp, const int *box, double *host);
 int cfd_read_id(cfd_sim *s, int16_t *host);                     /* Structure::ID */
 int cfd_port_npts(cfd_sim *s, int port);
 int cfd_port_points(cfd_sim *s, int port, int *ijk);
@@ -185,7 +189,7 @@ int cfd_model_table(cfd_model *m, const char *name, double *out);
 int cfd_model_tfsf(cfd_model *m, int h, int *targets, double *terms, int *nterms);
 int cfd_model_port(cfd_model *m, int port, int *ijk);               /* returns point count */
 /* DL / gain cell lists in reference [i][j][k] flat order; returns count */
-int cfd_model_cells(cfd_model *m, int gain, int *flat);
+int cfd_model_cells(cfd_model *m, int gain, long long *flat);
 
 /* Per-kernel timing of the last cfd_step call (HIP events on the sim's stream):
  * names[i] (<=32 chars) and total milliseconds; returns entries. */

@@ -69,3 +69,42 @@ def test_c2_full_run_f32_vs_f64(tmp_path):
         err_f = rel_l2(runs[32][1][q], runs[64][1][q])
         print("port %d: |E|^2 series rel L2 %.2e, DFT rel L2 %.2e" % (q, err_t, err_f), flush=True)
         assert err_t < 1e-4 and err_f < 1e-4, (q, err_t, err_f)
+
+
+def corner_case(directory, n, nt=10):
+    """PEC box of n^3 cells, dx = 10 nm, an eps_r 4 cuboid and an Ez point
+    source near the low corner, no CPML (every tile on the lean path)."""
+    from cpsfdtd_amd.case import CaseWriter, Shape, FIXED
+    d = (10e-9, 10e-9, 10e-9)
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (4.0, 1, 0, 0)])
+    w.grid((n, n, n), d, [Shape(1, "cuboid", x0=4 * d[0], y0=18 * d[1], z0=6 * d[2], a=6 * d[0], b=5 * d[1], c=8 * d[2])])
+    w.source(500e-9, 2e-15, 4e-15)
+    w.point_source(12, 14, 16, comp=2, amp=1.0)
+    return w.run(nt, flags=FIXED)
+
+
+def test_grid_beyond_2_31_cells(tmp_path):
+    """A 1300^3 FP32 grid (2.2e9 cells, 53 GB of fields: 64-bit cell indexing
+    everywhere) against a 48^3 grid with the same corner geometry and source:
+    after 10 steps influence has travelled at most 20 cells, so the corner
+    region [0,24)^3 of both grids is the same, bit for bit."""
+    import os
+    os.makedirs(tmp_path / "big")
+    os.makedirs(tmp_path / "small")
+    print("1300^3 FP32 grid, 10 steps", flush=True)
+    big = eng.Engine(corner_case(str(tmp_path / "big"), 1300), 32)
+    assert big.N == (1300, 1300, 1300) and 1301 ** 3 > 2 ** 31
+    big.step(10)
+    small = eng.Engine(corner_case(str(tmp_path / "small"), 48), 32)
+    small.step(10)
+    for c in range(6):
+        a = big.region(c, 0, 24, 0, 24, 0, 24)
+        b = small.region(c, 0, 24, 0, 24, 0, 24)
+        assert np.abs(b).max() > 0 or c in (3,), c
+        assert np.array_equal(a, b), c
+    # the far corner of the big grid is untouched
+    assert np.abs(big.region(2, 1270, 1300, 1270, 1300, 1270, 1300)).max() == 0.0
+    assert big.nonfinite() == 0
+    big.close()
+    small.close()
