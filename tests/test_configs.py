@@ -91,3 +91,55 @@ def test_c5_request_model_builds(tmp_path):
         d.mkdir()
         m = eng.Model(configs.c5(str(d), p))
         assert tuple(m.N) == p["N"] and len(m.port_points(0)) > 0
+
+
+def test_c5_media_clear_of_cpml(tmp_path):
+    """No request of the stream puts a non-vacuum cell inside the 10-cell
+    CPML (the reference's CPML E terms assume vacuum, CPML.cpp:524)."""
+    rq = configs.c5_requests(40)
+    kinds = set()
+    for k, (_, p) in enumerate(rq):
+        if p["N"][0] * p["N"][1] * p["N"][2] > 3e6:
+            continue
+        kinds.add(p["kind"])
+        d = tmp_path / ("r%d" % k)
+        d.mkdir()
+        idm = eng.Model(configs.c5(str(d), p)).id_map()
+        inner = idm[10:-10, 10:-10, 10:-10].copy()
+        assert (idm != 0).sum() == (inner != 0).sum() > 0, (k, p)
+    assert kinds == {"sphere", "dl", "gain"}
+
+
+def _slab_case(directory, inset, nt):
+    """A c5 gain request at 40x40x48 with its slab ``inset`` cells from the
+    side faces (0: through the side CPML)."""
+    from cpsfdtd_amd.case import CaseWriter, Shape, FIXED
+    nx, ny, nz, dx = 40, 40, 48, 10e-9
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0), (2.25, 1, 0, 0)])
+    h = nz // 4
+    w.grid((nx, ny, nz), (dx, dx, dx),
+           [Shape(5, "cuboid", x0=inset * dx, y0=inset * dx, z0=(nz // 2 - h // 2) * dx, a=(nx + 1 - 2 * inset) * dx,
+                  b=(ny + 1 - 2 * inset) * dx, c=h * dx)])
+    w.cpml(10, m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.source(600e-9, 10e-15, 30e-15)
+    w.point_source(nx // 2, ny // 2, nz // 2, comp=2, amp=1.0)
+    w.gain(600e-9, 1e14, 1e-13, 1e-10, 1e-13, 3.3e24, 1e9)
+    return w.run(nt)
+
+
+def test_dielectric_through_cpml_diverges_on_oracle(tmp_path, oracle_lib):
+    """The reference's arithmetic: the eps_r 2.25 gain slab of a c5 request
+    stays bounded when it ends 12 cells from the side faces, and grows
+    without bound when it runs through the side CPML (vacuum dt/eps0 on the
+    CPML E terms, CPML.cpp:524)."""
+    peak = {}
+    for inset in (12, 0):
+        d = tmp_path / ("i%d" % inset)
+        d.mkdir()
+        o = oracle_lib.OracleSim(_slab_case(str(d), inset, 400))
+        o.step(400)
+        with np.errstate(invalid="ignore"):
+            peak[inset] = max(float(np.nan_to_num(np.abs(o.field(c)), nan=np.inf).max()) for c in range(6))
+    assert peak[12] < 1e-4  # the source pulse is still rising (3.4e-6)
+    assert peak[0] > 1e6    # non-finite by then
