@@ -23,6 +23,22 @@ __device__ __forceinline__ long long cell_index(int px, int py, int i, int j, in
     return (long long)i + (long long)px * ((long long)j + (long long)py * k);
 }
 
+// A half-step reads the updated component at its own cell once and writes it
+// once; no other tile reads it in that pass.  Those loads and stores are
+// non-temporal, so they stream past the caches and leave room for the
+// neighbour values that adjacent tiles re-read (measured on the 256^3 FP32
+// workload: both merged kernels 4-5 % faster; stores alone change nothing).
+template <class T>
+__device__ __forceinline__ T ld_once(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_once(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // ---------------------------------------------------------------------------
 // per-cell physics, shared by the grid kernels (module passes) and the
 // z-marching kernels (fused step).  Neighbour values arrive as arguments.
@@ -316,9 +332,9 @@ __global__ __launch_bounds__(TX *TY) void k_update_h(Dev<R> d, unsigned parts, T
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     h_cell(d, (parts & H_BASIC) != 0, pml ? 7u : 0u, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
            d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
-    d.F[3][p] = hx;
-    d.F[4][p] = hy;
-    d.F[5][p] = hz;
+    st_once(&d.F[3][p], hx);
+    st_once(&d.F[4][p], hy);
+    st_once(&d.F[5][p], hz);
 }
 
 template <class R>
@@ -342,9 +358,9 @@ __global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts, T
     R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
     e_cell(d, parts, pml ? 7u : 0u, i, j, k, p, id, d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm], d.F[3][pkm],
            d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
-    d.F[0][p] = E[0];
-    d.F[1][p] = E[1];
-    d.F[2][p] = E[2];
+    st_once(&d.F[0][p], E[0]);
+    st_once(&d.F[1][p], E[1]);
+    st_once(&d.F[2][p], E[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,9 +439,9 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<
     hx = aH * hx - bH * (d.F[2][pj1] - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (d.F[1][pk1] - ey) * lean_st<CS>(d, c, S_cHxy, k);
     hy = aH * hy - bH * (d.F[0][pk1] - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (d.F[2][pi1] - ez) * lean_st<CS>(d, c, S_cHyz, i);
     hz = aH * hz + bH * (d.F[0][pj1] - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (d.F[1][pi1] - ey) * lean_st<CS>(d, c, S_cHzy, i);
-    d.F[3][p] = hx;
-    d.F[4][p] = hy;
-    d.F[5][p] = hz;
+    st_once(&d.F[3][p], hx);
+    st_once(&d.F[4][p], hy);
+    st_once(&d.F[5][p], hz);
 }
 
 // interior E: BasicFDTD.cpp:25-45 for plain cells (ID 4 zeroed; ID 2 and, with a
@@ -476,9 +492,9 @@ __global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, BoxSet S)
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     h_cell(d, true, (unsigned)S.b[bi].pml, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
            d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
-    d.F[3][p] = hx;
-    d.F[4][p] = hy;
-    d.F[5][p] = hz;
+    st_once(&d.F[3][p], hx);
+    st_once(&d.F[4][p], hy);
+    st_once(&d.F[5][p], hz);
 }
 
 template <class R>
@@ -491,9 +507,9 @@ __device__ __forceinline__ void e_full(const Dev<R> &d, int i, int j, int k, uns
     R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
     e_cell(d, E_ALL, pm, i, j, k, p, d.id[p], d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm],
            d.F[3][pkm], d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
-    d.F[0][p] = E[0];
-    d.F[1][p] = E[1];
-    d.F[2][p] = E[2];
+    st_once(&d.F[0][p], E[0]);
+    st_once(&d.F[1][p], E[1]);
+    st_once(&d.F[2][p], E[2]);
 }
 
 // CPML shell, E: stencil and psi only.  Drude-Lorentz and gain cells of the
@@ -508,7 +524,7 @@ __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
     const int id = d.id[p];
-    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    R E[3] = {ld_once(&d.F[0][p]), ld_once(&d.F[1][p]), ld_once(&d.F[2][p])};
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
@@ -519,9 +535,9 @@ __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
     e_cell(d, E_BASIC | E_PML, sp ? 0u : (unsigned)S.b[bi].pml, i, j, k, p, id, hx, hy, hz, hx_jm, hz_jm, hx_km,
            hy_km, hy_im, hz_im, E);
     for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
-    d.F[0][p] = E[0];
-    d.F[1][p] = E[1];
-    d.F[2][p] = E[2];
+    st_once(&d.F[0][p], E[0]);
+    st_once(&d.F[1][p], E[1]);
+    st_once(&d.F[2][p], E[2]);
 }
 
 // Batched launches: one launch covers every instance of a batch (gridDim.y
@@ -572,7 +588,7 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
     const R ex_j1 = d.F[0][pj1], ez_j1 = d.F[2][pj1], ex_k1 = d.F[0][pk1];
     const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
-    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    R hx = ld_once(&d.F[3][p]), hy = ld_once(&d.F[4][p]), hz = ld_once(&d.F[5][p]);
     if (tile_inside(L, x0, y0, k)) {
         const int id = d.id[p];
         const R aH = d.aH[id], bH = d.bH[id];
@@ -583,9 +599,9 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
         h_cell(d, true, d.has_pml ? tile_pml_mask(d, x0, y0, k, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
                ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz);
     }
-    d.F[3][p] = hx;
-    d.F[4][p] = hy;
-    d.F[5][p] = hz;
+    st_once(&d.F[3][p], hx);
+    st_once(&d.F[4][p], hy);
+    st_once(&d.F[5][p], hz);
 }
 
 template <class R, bool CS, bool B = false>
@@ -602,7 +618,7 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
     const int id = d.id[p];
-    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    R E[3] = {ld_once(&d.F[0][p]), ld_once(&d.F[1][p]), ld_once(&d.F[2][p])};
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
@@ -620,9 +636,9 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
                hz, hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E);
         for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
     }
-    d.F[0][p] = E[0];
-    d.F[1][p] = E[1];
-    d.F[2][p] = E[2];
+    st_once(&d.F[0][p], E[0]);
+    st_once(&d.F[1][p], E[1]);
+    st_once(&d.F[2][p], E[2]);
 }
 
 // Drude-Lorentz and gain cells, all of them: device cell index, with bit
