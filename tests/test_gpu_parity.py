@@ -100,8 +100,12 @@ def test_dl_gain_module_f64_bit_exact(tmp_path):
     assert np.array_equal(e.gain_pop(), o.gain_pop())
 
 
+@pytest.mark.parametrize("form", ["split", "merged"])
 @pytest.mark.parametrize("periodic,npml", [(1, (0, 5, 6)), (3, (0, 0, 6))], ids=["x", "xy"])
-def test_periodic_f64_bit_exact(tmp_path, periodic, npml):
+def test_periodic_f64_bit_exact(tmp_path, monkeypatch, periodic, npml, form):
+    """Periodic x (and y) planes, in the split form and in the merged form
+    (whose epilogue applies the TF/SF terms on the periodic E planes too)."""
+    monkeypatch.setenv("CFD_UPDATE_KERNEL", form)
     case = cases.mixed(str(tmp_path), periodic=periodic, npml=npml, flags=FIXED, theta=0.0, phi=0.0, psi=0.0)
     e, o = run_both(case, 50)
     assert_fields_equal(e, o, "periodic")
