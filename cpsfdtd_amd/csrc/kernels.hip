@@ -46,6 +46,16 @@ __device__ __forceinline__ void st_once(T *p, T v)
 
 // CPML term of one axis at one cell.  The two slabs of an axis never overlap
 // (the model rejects n_low + n_high >= N), so at most one holds the cell.
+// A cell of the special pass (Drude-Lorentz or gain), in bitwise form: a
+// short-circuit || becomes a branch, and the wait on the ID byte it needs
+// then lands ahead of every later load (measured: the E CPML tiles issued
+// their psi loads one memory round trip late).
+template <class R>
+__device__ __forceinline__ bool special_id(const Dev<R> &d, int id)
+{
+    return ((id == 2) & (d.ndl != 0)) | ((id == 5) & (d.ngain != 0));
+}
+
 template <class R>
 struct SlabTerm {
     R b, c, q1, q2;  // grading coefficients and the two psi values
@@ -111,11 +121,12 @@ __device__ __forceinline__ void slab_store(const Dev<R> &d, int f, int ax, const
 template <class R>
 __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm, int i, int j, int k, long long p,
                                        R ex, R ey, R ez, R ex_j1, R ez_j1, R ex_k1, R ey_k1, R ey_i1, R ez_i1,
-                                       R &hx, R &hy, R &hz)
+                                       R &hx, R &hy, R &hz, int id_loaded = -1)
 {
-    // ID and stretch loads first, then the slab terms: every load of the cell
-    // is in flight before the first wait
-    const int id = basic ? (int)d.id[p] : 0;
+    // ID (unless the caller loaded it ahead of its field loads) and stretch
+    // loads first, then the slab terms: every load of the cell is in flight
+    // before the first wait
+    const int id = basic ? (id_loaded >= 0 ? id_loaded : (int)d.id[p]) : 0;
     R sxz = 0, sxy = 0, syx = 0, syz = 0, szx = 0, szy = 0;
     if (basic) {
         sxz = d.st[S_cHxz][j]; sxy = d.st[S_cHxy][k]; syx = d.st[S_cHyx][k];
@@ -164,7 +175,7 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
 template <class R>
 __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned pm, int i, int j, int k, long long p,
                                        int id, R hx, R hy, R hz, R hx_jm, R hz_jm, R hx_km, R hy_km, R hy_im,
-                                       R hz_im, R (&E)[3])
+                                       R hz_im, R (&E)[3], bool skip_special = false)
 {
     // stretch values and slab terms loaded ahead of every store below
     R sxz = 0, sxy = 0, syx = 0, syz = 0, szy = 0, szx = 0;
@@ -193,15 +204,12 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
             }
         }
     }
-    if (parts & E_BASIC) {
-        if (id == 4) {
-            E[0] = 0; E[1] = 0; E[2] = 0;
-        } else if (id != 2) {
-            const R aE = d.aE[id], bE = d.bE[id];
-            E[0] = aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy;
-            E[1] = aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz;
-            E[2] = aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx;
-        }
+    if (parts & E_BASIC) {  // branch-free: ID 4 zeroed, ID 2 left as is
+        const R aE = d.aE[id], bE = d.bE[id];
+        const R u[3] = {aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy,
+                        aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz,
+                        aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx};
+        for (int c = 0; c < 3; c++) E[c] = id == 4 ? (R)0 : id == 2 ? E[c] : u[c];
     }
     if (dl) {
         const int s = d.aux[p];
@@ -257,25 +265,28 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
         }
     }
     if (!pm) return;
+    // with skip_special, the special pass's cells keep their psi (that pass
+    // updates it); tested here, after every load above has been issued
+    const bool psi_store = !(skip_special && special_id(d, id));
     const R cE = d.cpmlE;
     if (T[0].side >= 0) {  // x slab: Ey -= psi(dHz/dx), Ez += psi(dHy/dx)
         const R a1 = T[0].b * T[0].q1 + T[0].c * (hz - hz_im);
         const R a2 = T[0].b * T[0].q2 + T[0].c * (hy - hy_im);
-        slab_store(d, 0, 0, T[0], a1, a2);
+        if (psi_store) slab_store(d, 0, 0, T[0], a1, a2);
         E[1] = E[1] - a1 * cE;
         E[2] = E[2] + a2 * cE;
     }
     if (T[1].side >= 0) {  // y slab: Ez -= psi(dHx/dy), Ex += psi(dHz/dy)
         const R a1 = T[1].b * T[1].q1 + T[1].c * (hx - hx_jm);
         const R a2 = T[1].b * T[1].q2 + T[1].c * (hz - hz_jm);
-        slab_store(d, 0, 1, T[1], a1, a2);
+        if (psi_store) slab_store(d, 0, 1, T[1], a1, a2);
         E[2] = E[2] - a1 * cE;
         E[0] = E[0] + a2 * cE;
     }
     if (T[2].side >= 0) {  // z slab: Ex -= psi(dHy/dz), Ey += psi(dHx/dz)
         const R a1 = T[2].b * T[2].q1 + T[2].c * (hy - hy_km);
         const R a2 = T[2].b * T[2].q2 + T[2].c * (hx - hx_km);
-        slab_store(d, 0, 2, T[2], a1, a2);
+        if (psi_store) slab_store(d, 0, 2, T[2], a1, a2);
         E[0] = E[0] - a1 * cE;
         E[1] = E[1] + a2 * cE;
     }
@@ -432,9 +443,9 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<
     const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
     const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
     const long long pk1 = p + d.slab;
+    const int id = d.id[p];  // first: the material lookup then overlaps the field loads
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    const int id = d.id[p];
     const R aH = d.aH[id], bH = d.bH[id];
     hx = aH * hx - bH * (d.F[2][pj1] - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (d.F[1][pk1] - ey) * lean_st<CS>(d, c, S_cHxy, k);
     hy = aH * hy - bH * (d.F[0][pk1] - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (d.F[2][pi1] - ez) * lean_st<CS>(d, c, S_cHyz, i);
@@ -470,7 +481,7 @@ __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<
     R e0 = aE * ex + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
     R e1 = aE * ey + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
     R e2 = aE * ez + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
-    const bool keep = id == 2 || (skip5 && id == 5);
+    const bool keep = (id == 2) | ((skip5 != 0) & (id == 5));
     e0 = keep ? ex : id == 4 ? (R)0 : e0;
     e1 = keep ? ey : id == 4 ? (R)0 : e1;
     e2 = keep ? ez : id == 4 ? (R)0 : e2;
@@ -489,9 +500,10 @@ __global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, BoxSet S)
     const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
     const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
     const long long pk1 = p + d.slab;
+    const int id = d.id[p];  // first: the material lookup then overlaps the field loads
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     h_cell(d, true, (unsigned)S.b[bi].pml, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
-           d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
+           d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz, id);
     st_once(&d.F[3][p], hx);
     st_once(&d.F[4][p], hy);
     st_once(&d.F[5][p], hz);
@@ -528,12 +540,12 @@ __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
-    // special-pass cells: no psi update here, own values stored back (no
-    // branch on the ID ahead of the loads, as in k_lean_e)
-    const bool sp = (id == 2 && d.ndl) || (id == 5 && d.ngain);
+    // special-pass cells: no psi update here, own values stored back (the ID
+    // is tested after the loads are issued, as in k_lean_e)
     const R E0[3] = {E[0], E[1], E[2]};
-    e_cell(d, E_BASIC | E_PML, sp ? 0u : (unsigned)S.b[bi].pml, i, j, k, p, id, hx, hy, hz, hx_jm, hz_jm, hx_km,
-           hy_km, hy_im, hz_im, E);
+    e_cell(d, E_BASIC | E_PML, (unsigned)S.b[bi].pml, i, j, k, p, id, hx, hy, hz, hx_jm, hz_jm, hx_km, hy_km, hy_im,
+           hz_im, E, true);
+    const bool sp = special_id(d, id);
     for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
     st_once(&d.F[0][p], E[0]);
     st_once(&d.F[1][p], E[1]);
@@ -585,19 +597,21 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
     const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
     const long long pj1 = (d.pby && j + 1 == d.Ny) ? p + d.px - (long long)d.Ny * d.px : p + d.px;
     const long long pk1 = p + d.slab;
+    // the ID byte first: loads return in issue order, so the material lookup
+    // that depends on it then overlaps the field loads
+    const int id = d.id[p];
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
     const R ex_j1 = d.F[0][pj1], ez_j1 = d.F[2][pj1], ex_k1 = d.F[0][pk1];
     const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
     R hx = ld_once(&d.F[3][p]), hy = ld_once(&d.F[4][p]), hz = ld_once(&d.F[5][p]);
     if (tile_inside(L, x0, y0, k)) {
-        const int id = d.id[p];
         const R aH = d.aH[id], bH = d.bH[id];
         hx = aH * hx - bH * (ez_j1 - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (ey_k1 - ey) * lean_st<CS>(d, c, S_cHxy, k);
         hy = aH * hy - bH * (ex_k1 - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (ez_i1 - ez) * lean_st<CS>(d, c, S_cHyz, i);
         hz = aH * hz + bH * (ex_j1 - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (ey_i1 - ey) * lean_st<CS>(d, c, S_cHzy, i);
     } else {
         h_cell(d, true, d.has_pml ? tile_pml_mask(d, x0, y0, k, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
-               ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz);
+               ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz, id);
     }
     st_once(&d.F[3][p], hx);
     st_once(&d.F[4][p], hy);
@@ -622,18 +636,21 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
-    const bool sp = (id == 2 && d.ndl) || (id == 5 && d.ngain);  // the special pass's cells
+    // the special pass's cells keep their values here; the ID test sits in
+    // each branch after its loads (hoisted ahead of the branch, it made the
+    // CPML tiles wait for the ID byte before issuing their psi loads)
     const R E0[3] = {E[0], E[1], E[2]};
     if (tile_inside(L, x0, y0, k)) {
         const R aE = d.aE[id], bE = d.bE[id];
         E[0] = aE * E[0] + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
         E[1] = aE * E[1] + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
         E[2] = aE * E[2] + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
-        const bool keep = sp || id == 2;  // ID 2 without the ADE module is left as is
+        const bool keep = special_id(d, id) | (id == 2);  // ID 2 without the ADE module is left as is
         for (int q = 0; q < 3; q++) E[q] = keep ? E0[q] : id == 4 ? (R)0 : E[q];
     } else {
-        e_cell(d, E_BASIC | E_PML, sp || !d.has_pml ? 0u : tile_pml_mask(d, x0, y0, k, true), i, j, k, p, id, hx, hy,
-               hz, hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E);
+        e_cell(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, x0, y0, k, true) : 0u, i, j, k, p, id, hx, hy, hz,
+               hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E, true);
+        const bool sp = special_id(d, id);
         for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
     }
     st_once(&d.F[0][p], E[0]);
