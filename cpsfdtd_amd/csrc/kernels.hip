@@ -40,8 +40,9 @@ __device__ __forceinline__ void st_once(T *p, T v)
 }
 
 // ---------------------------------------------------------------------------
-// per-cell physics, shared by the grid kernels (module passes) and the
-// z-marching kernels (fused step).  Neighbour values arrive as arguments.
+// per-cell physics, shared by the grid kernels (module passes), the CPML
+// shell and merged kernels (fused step) and the special pass.  Neighbour
+// values arrive as arguments.
 // ---------------------------------------------------------------------------
 
 // CPML term of one axis at one cell.  The two slabs of an axis never overlap
