@@ -270,6 +270,33 @@ def serving_c5(nreq=120, precision=32, quantum=256):
             "peak_device_gb": round(stats["peak_bytes"] / 1e9, 2), "quantum": quantum}
 
 
+def other_configs(steps=400, warmup=20):
+    """The single-grid workloads beside c2, each in FP32 and FP64: c1 (128^3
+    sphere, Ez source, line port recording every step) and c3 (192x192x64
+    rod slab with four-level gain, periodic x/y, sphere port), ``steps``
+    timed steps after ``warmup``, Mcell-updates/s (Nx Ny Nz steps / wall)."""
+    import cpsfdtd_amd as eng
+    from cpsfdtd_amd import configs
+    out = {}
+    for name, make in (("c1", configs.c1), ("c3", configs.c3)):
+        case = make(tempfile.mkdtemp(prefix="bench_%s_" % name), nt=warmup + steps + 1)
+        N = eng.Model(case).N
+        cells = N[0] * N[1] * N[2]
+        row = {"cells": cells, "steps": steps}
+        for prec in (32, 64):
+            e = eng.Engine(case, prec)
+            e.step(warmup)
+            e.sync()
+            t0 = time.perf_counter()
+            e.step(steps)
+            e.sync()
+            wall = time.perf_counter() - t0
+            e.close()
+            row["f%d" % prec] = round(cells * steps / wall / 1e6, 1)
+        out[name] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +307,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched c4 measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served c5 measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the c1 / c3 measurements")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--copy-calib", action="store_true",
@@ -381,8 +409,16 @@ def main():
                        "kernels": {k: round(v["avg_ms"], 5) for k, v in r64["kernels"].items()}}
         out["fp32_over_fp64"] = round(r32["mcells"] / r64["mcells"], 3)
         out["quality_f32_vs_f64"] = quality(r32["results"], r64["results"], case.freqs)
-    # the batch and serving figures are reported beside the main one; a
+    # the other workloads' figures are reported beside the main one; a
     # failure there is recorded in the line instead of losing it
+    if not args.no_configs:
+        log("bench: c1 and c3, FP32 and FP64")
+        try:
+            out["configs"] = other_configs()
+            log("bench: %s" % out["configs"])
+        except Exception as ex:
+            out["configs"] = {"error": str(ex)}
+            log("bench: c1/c3 failed: %s" % ex)
     if not args.no_batch:
         log("bench: batched c4 (512 unit cells)")
         try:
