@@ -23,7 +23,8 @@ FIX_CPML_HSIGN = 1     # CPML.cpp:643-682 H-side y/z psi sign -> Roden-Gedney
 FIX_TFSF_FARFACE = 2   # TF_SF.cpp:1222,1255,1288 far-face H index
 SRC_REF_PULSE = 4      # Sources.cpp:72-76 Pulse() waveform for point sources
 REF_ZEROHEAP = 8       # reproduce the reference on zero-filled heap memory
-FIXED = FIX_CPML_HSIGN | FIX_TFSF_FARFACE
+FIX_TFSF_YEE = 16      # TF_SF.cpp:155-887 incident values at each component's Yee position
+FIXED = FIX_CPML_HSIGN | FIX_TFSF_FARFACE | FIX_TFSF_YEE
 
 PORT_PLANE = 0
 PORT_SPHERE = 1

@@ -440,19 +440,25 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
                 if (!m.pbx) for (int j = j1; j < j2; ++j) recs.push_back({i2, j, c0, 2});
             }
             for (const Rec &r : recs) {
-                double x = m.dx * (r.i - i0), y = m.dy * (r.j - j0), z = m.dz * (r.k - k0);
-                double xe = x, ye = y, ze = z;
-                if (a == 0) xe = m.dx * (r.i - i0 + sh);
-                if (a == 1) ye = m.dy * (r.j - j0 + sh);
-                if (a == 2) ze = m.dz * (r.k - k0 + sh);
-                double dE, pE, dH, pH;
-                at(xe, ye, ze, dE, pE);
-                at(x, y, z, dH, pH);
                 int hi = r.i, hj = r.j, hk = r.k;
                 const int off = e ? far : -1;
                 if (a == 0) hi += off; else if (a == 1) hj += off; else hk += off;
                 for (int c = 0; c < 2; c++) {
                     if (!(r.part & (1 << c))) continue;
+                    // incident values of this component pair: the H one (for the E
+                    // correction) half a cell across the face, the E one on it; with
+                    // CFD_FIX_TFSF_YEE both also half a cell along the E component's
+                    // own axis, their Yee position (the reference: the cell corner)
+                    double o[3] = {0, 0, 0};
+                    if (m.flags & CFD_FIX_TFSF_YEE) o[compE[a][c]] = 0.5;
+                    const double x = m.dx * (r.i - i0 + o[0]), y = m.dy * (r.j - j0 + o[1]), z = m.dz * (r.k - k0 + o[2]);
+                    double xe = x, ye = y, ze = z;
+                    if (a == 0) xe = m.dx * (r.i - i0 + sh);
+                    if (a == 1) ye = m.dy * (r.j - j0 + sh);
+                    if (a == 2) ze = m.dz * (r.k - k0 + sh);
+                    double dE, pE, dH, pH;
+                    at(xe, ye, ze, dE, pE);
+                    at(x, y, z, dH, pH);
                     const double s = sgn[a][c] * flip;
                     add(false, compE[a][c], r.i, r.j, r.k, s < 0 ? -ampE[a][c] : ampE[a][c], dE, pE);
                     const double ph = ((m.flags & CFD_REF_ZEROHEAP) && a == 0 && r.part != 3) ? 0.0 : pH;

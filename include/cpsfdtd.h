@@ -28,6 +28,7 @@ extern "C" {
 #define CFD_FIX_TFSF_FARFACE 2u  /* TF_SF.cpp:1222,1255,1288 far-face H at index+0 */
 #define CFD_SRC_REF_PULSE    4u  /* point sources follow Sources.cpp:72-76 Pulse() */
 #define CFD_REF_ZEROHEAP     8u  /* reproduce the reference's reads of zero-filled unset memory */
+#define CFD_FIX_TFSF_YEE    16u  /* TF_SF.cpp:155-887 incident values at each component's Yee position */
 
 #define CFD_PORT_PLANE  0
 #define CFD_PORT_SPHERE 1

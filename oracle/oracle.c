@@ -20,6 +20,8 @@
  *   F15 TF/SF far-face H written at index+1        -> reference index unless ORC_FIX_TFSF_FARFACE
  *   F16 TF/SF x-face H edge loops read the unset   -> corrected (Hz with the Ey-edge table at
  *       table half and update the wrong component      k=k2, Hy with the Ez-edge table at j=j2)
+ *   F18 TF/SF incident values at the cell corner,  -> reference positions unless ORC_FIX_TFSF_YEE
+ *       not at each component's Yee position
  * Shapes that reach outside the grid are clipped (the reference writes out of
  * bounds, Structure.cpp:166,231).
  */
@@ -119,7 +121,9 @@ typedef struct {
     double cpmlE, cpmlH;
 } cpml_t;
 
-typedef struct { int i, j, k, part; double dE, pE, dH, pH; } tfsf_pt; /* part: 3 main, 1 comp1, 2 comp2 */
+/* part: 3 main, 1 comp1, 2 comp2; [c]: incident delay/profile for the E and
+ * the H correction of component pair c (TF_SF.cpp's *_E1/_H1, *_E2/_H2) */
+typedef struct { int i, j, k, part; double dE[2], pE[2], dH[2], pH[2]; } tfsf_pt;
 
 typedef struct {
     int ok;
@@ -831,15 +835,22 @@ static void tfsf_at(const tfsf_t *T, double dt, double x, double y, double z, do
 
 static void tfsf_point(orc_sim *s, tfsf_t *T, int i, int j, int k, int part, int axis, double shiftE)
 {
+    /* the E components a face of this axis corrects (record parts 1, 2) */
+    static const int comp[3][2] = {{1, 2}, {0, 2}, {0, 1}};
     tfsf_pt *q = &T->pt[T->npts++];
     q->i = i; q->j = j; q->k = k; q->part = part;
-    double x = s->dx * (i - T->i0), y = s->dy * (j - T->j0), z = s->dz * (k - T->k0);
-    double xe = x, ye = y, ze = z;
-    if (axis == 0) xe = s->dx * (i - T->i0 + shiftE);
-    if (axis == 1) ye = s->dy * (j - T->j0 + shiftE);
-    if (axis == 2) ze = s->dz * (k - T->k0 + shiftE);
-    tfsf_at(T, s->dt, xe, ye, ze, &q->dE, &q->pE);
-    tfsf_at(T, s->dt, x, y, z, &q->dH, &q->pH);
+    for (int c = 0; c < 2; c++) {
+        /* ORC_FIX_TFSF_YEE: the pair's Yee offset along its E component's axis */
+        double o[3] = {0, 0, 0};
+        if (s->flags & ORC_FIX_TFSF_YEE) o[comp[axis][c]] = 0.5;
+        double x = s->dx * (i - T->i0 + o[0]), y = s->dy * (j - T->j0 + o[1]), z = s->dz * (k - T->k0 + o[2]);
+        double xe = x, ye = y, ze = z;
+        if (axis == 0) xe = s->dx * (i - T->i0 + shiftE);
+        if (axis == 1) ye = s->dy * (j - T->j0 + shiftE);
+        if (axis == 2) ze = s->dz * (k - T->k0 + shiftE);
+        tfsf_at(T, s->dt, xe, ye, ze, &q->dE[c], &q->pE[c]);
+        tfsf_at(T, s->dt, x, y, z, &q->dH[c], &q->pH[c]);
+    }
 }
 
 /* TF_SF.cpp:29-959 */
@@ -997,26 +1008,30 @@ static void tfsf_apply(orc_sim *s, int n, int is_h)
             for (int q = T->face_start[f]; q < T->face_start[f + 1]; q++) {
                 const tfsf_pt *pt = &T->pt[q];
                 int i = pt->i, j = pt->j, k = pt->k;
-                double inc, prof, *c2;
+                double *c2;
                 double (*sg)[2];
                 double *(*tg)[2];
                 if (is_h) {
                     int off = e ? far : -1;
                     if (a == 0) i += off; else if (a == 1) j += off; else k += off;
-                    inc = interp(T->Einc, (double)n - pt->dH);
-                    prof = pt->pH;
                     tg = tH; c2 = cH[a]; sg = sH;
-                    /* zero-heap emulation: the reference's x-face H edge loops read
-                     * the unset half of the point record (F16) */
-                    if ((s->flags & ORC_REF_ZEROHEAP) && a == 0 && pt->part != 3) prof = 0.0;
                 } else {
-                    inc = interp(T->Hinc, (double)n - pt->dE);
-                    prof = pt->pE;
                     tg = tE; c2 = cE[a]; sg = sE;
                 }
                 size_t p = IDX(s, i, j, k);
                 for (int c = 0; c < 2; c++) {
                     if (!(pt->part & (1 << c))) continue;
+                    double inc, prof;
+                    if (is_h) {
+                        inc = interp(T->Einc, (double)n - pt->dH[c]);
+                        prof = pt->pH[c];
+                        /* zero-heap emulation: the reference's x-face H edge loops read
+                         * the unset half of the point record (F16) */
+                        if ((s->flags & ORC_REF_ZEROHEAP) && a == 0 && pt->part != 3) prof = 0.0;
+                    } else {
+                        inc = interp(T->Hinc, (double)n - pt->dE[c]);
+                        prof = pt->pE[c];
+                    }
                     double *F = tg[a][c];
                     if (sg[a][c] * flip < 0) F[p] = F[p] - c2[c] * inc * prof;
                     else F[p] = F[p] + c2[c] * inc * prof;
@@ -1404,7 +1419,11 @@ int orc_get_tfsf_points(const orc_sim *s, int *ijk, double *tab)
     for (int q = 0; q < s->tf.npts && (ijk || tab); q++) {
         const tfsf_pt *p = &s->tf.pt[q];
         if (ijk) { ijk[4 * q] = p->i; ijk[4 * q + 1] = p->j; ijk[4 * q + 2] = p->k; ijk[4 * q + 3] = p->part; }
-        if (tab) { tab[4 * q] = p->dE; tab[4 * q + 1] = p->pE; tab[4 * q + 2] = p->dH; tab[4 * q + 3] = p->pH; }
+        if (tab)
+            for (int c = 0; c < 2; c++) {
+                double *t = tab + 8 * q + 4 * c;
+                t[0] = p->dE[c]; t[1] = p->pE[c]; t[2] = p->dH[c]; t[3] = p->pH[c];
+            }
     }
     return s->tf.npts;
 }

@@ -28,6 +28,11 @@ extern "C" {
  * (TF_SF.cpp:1244-1267), which read unset point fields, add nothing.  Used only
  * to pin the oracle bit-for-bit against the compiled reference. */
 #define ORC_REF_ZEROHEAP     8u
+/* TF_SF.cpp:155-887 takes every incident value at the cell corner (i, j, k),
+ * shifted half a cell along the face normal only; set: at the Yee position of
+ * the field component that supplies it (plus half a cell along the corrected
+ * E component's own axis, for the E and the H correction of that pair). */
+#define ORC_FIX_TFSF_YEE    16u
 
 #define ORC_PORT_PLANE  0
 #define ORC_PORT_SPHERE 1
@@ -75,7 +80,8 @@ int orc_get_stretch(const orc_sim *s, int which, double *out);
 /* CPML b/c: which = axis*4 + side*2 + (0:E,1:H); returns length; b,c may be NULL */
 int orc_get_cpml(const orc_sim *s, int which, double *b, double *c);
 /* TF/SF face points in construction order (TF_SF.cpp:147-887).
- * Returns count.  ijk: 3 ints; tab: dE pE dH pH per point (4 doubles). */
+ * Returns count.  ijk: i j k part per point (4 ints); tab: dE pE dH pH of
+ * component pair 1, then of pair 2, per point (8 doubles). */
 int orc_get_tfsf_points(const orc_sim *s, int *ijk, double *tab);
 int orc_get_tfsf_wave(const orc_sim *s, double *einc, double *hinc); /* returns Nt+1 */
 void orc_get_tfsf_vec(const orc_sim *s, double *e0h0_ce_ch); /* 12 doubles */

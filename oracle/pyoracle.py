@@ -195,7 +195,7 @@ class OracleSim:
     def tfsf_points(self):
         n = lib().orc_get_tfsf_points(self.h, None, None)
         ijk = np.empty((max(n, 1), 4), np.int32)
-        tab = np.empty((max(n, 1), 4), np.float64)
+        tab = np.empty((max(n, 1), 8), np.float64)  # per record: pair 1 (dE pE dH pH), pair 2
         lib().orc_get_tfsf_points(self.h, _ip(ijk), _dp(tab))
         return ijk[:n], tab[:n]
 

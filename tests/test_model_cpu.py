@@ -51,11 +51,12 @@ def build_expected(o, h, flags, face_id):
                 continue
             s = SIGN[a][c] * flip
             coef = -amp[a][c] if s < 0 else amp[a][c]
+            t = tab[q, 4 * c:4 * c + 4]  # this pair's dE pE dH pH
             if h:
-                prof = 0.0 if ((flags & REF_ZEROHEAP) and a == 0 and part != 3) else tab[q, 3]
-                term = (coef, tab[q, 2], prof)
+                prof = 0.0 if ((flags & REF_ZEROHEAP) and a == 0 and part != 3) else t[3]
+                term = (coef, t[2], prof)
             else:
-                term = (coef, tab[q, 0], tab[q, 1])
+                term = (coef, t[0], t[1])
             key = (comps[a][c], i, j, k)
             if key not in tg:
                 tg[key] = []

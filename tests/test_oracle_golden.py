@@ -89,8 +89,8 @@ def test_mixed_tables(oracle_lib, tmp_path, tag, kw):
     rt = g[p + "tfsf_tab"].reshape(-1, 8)
     assert np.array_equal(ijk[:, :3], rij)
     m1, m2 = (ijk[:, 3] & 1) > 0, (ijk[:, 3] & 2) > 0
-    assert np.array_equal(tab[m1], rt[m1][:, 0:4])
-    assert np.array_equal(tab[m2], rt[m2][:, 4:8])
+    assert np.array_equal(tab[m1][:, 0:4], rt[m1][:, 0:4])
+    assert np.array_equal(tab[m2][:, 4:8], rt[m2][:, 4:8])
     e, h = o.tfsf_wave()
     n = len(g[p + "tfsf_einc"])
     assert np.array_equal(e[:n], g[p + "tfsf_einc"]) and np.array_equal(h[:n], g[p + "tfsf_hinc"])
