@@ -27,7 +27,8 @@ __device__ __forceinline__ long long cell_index(int px, int py, int i, int j, in
 // once; no other tile reads it in that pass.  Those loads and stores are
 // non-temporal, so they stream past the caches and leave room for the
 // neighbour values that adjacent tiles re-read (measured on the 256^3 FP32
-// workload: both merged kernels 4-5 % faster; stores alone change nothing).
+// workload: both merged kernels 4-5 % faster, stores alone change nothing;
+// FP64 split form 4.5 % faster).
 template <class T>
 __device__ __forceinline__ T ld_once(const T *p)
 {
@@ -45,8 +46,6 @@ __device__ __forceinline__ void st_once(T *p, T v)
 // values arrive as arguments.
 // ---------------------------------------------------------------------------
 
-// CPML term of one axis at one cell.  The two slabs of an axis never overlap
-// (the model rejects n_low + n_high >= N), so at most one holds the cell.
 // A cell of the special pass (Drude-Lorentz or gain), in bitwise form: a
 // short-circuit || becomes a branch, and the wait on the ID byte it needs
 // then lands ahead of every later load (measured: the E CPML tiles issued
@@ -57,6 +56,8 @@ __device__ __forceinline__ bool special_id(const Dev<R> &d, int id)
     return ((id == 2) & (d.ndl != 0)) | ((id == 5) & (d.ngain != 0));
 }
 
+// CPML term of one axis at one cell.  The two slabs of an axis never overlap
+// (the model rejects n_low + n_high >= N), so at most one holds the cell.
 template <class R>
 struct SlabTerm {
     R b, c, q1, q2;  // grading coefficients and the two psi values
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<
     const long long pk1 = p + d.slab;
     const int id = d.id[p];  // first: the material lookup then overlaps the field loads
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
-    R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    R hx = ld_once(&d.F[3][p]), hy = ld_once(&d.F[4][p]), hz = ld_once(&d.F[5][p]);
     const R aH = d.aH[id], bH = d.bH[id];
     hx = aH * hx - bH * (d.F[2][pj1] - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (d.F[1][pk1] - ey) * lean_st<CS>(d, c, S_cHxy, k);
     hy = aH * hy - bH * (d.F[0][pk1] - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (d.F[2][pi1] - ez) * lean_st<CS>(d, c, S_cHyz, i);
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<
     // special pass store their own value back unchanged.
     const int id = d.id[p];
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
+    const R ex = ld_once(&d.F[0][p]), ey = ld_once(&d.F[1][p]), ez = ld_once(&d.F[2][p]);
     const R hz_jm = d.F[5][pjm], hy_km = d.F[4][pkm], hx_km = d.F[3][pkm];
     const R hz_im = d.F[5][pim], hy_im = d.F[4][pim], hx_jm = d.F[3][pjm];
     const R aE = d.aE[id], bE = d.bE[id];
@@ -486,9 +487,9 @@ __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<
     e0 = keep ? ex : id == 4 ? (R)0 : e0;
     e1 = keep ? ey : id == 4 ? (R)0 : e1;
     e2 = keep ? ez : id == 4 ? (R)0 : e2;
-    d.F[0][p] = e0;
-    d.F[1][p] = e1;
-    d.F[2][p] = e2;
+    st_once(&d.F[0][p], e0);
+    st_once(&d.F[1][p], e1);
+    st_once(&d.F[2][p], e2);
 }
 
 // CPML slab boxes: full physics, slab tests limited to the box's axes
