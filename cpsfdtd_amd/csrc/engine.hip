@@ -95,6 +95,33 @@ inline void sig_add(uint64_t &h, const std::vector<T> &v) {
     if (n) sig_add(h, v.data(), n * sizeof(T));
 }
 
+// Fields larger than this are streamed (non-temporal own-cell loads and
+// stores in the update kernels); smaller field sets stay in the MI355X's
+// 256 MiB Infinity Cache between passes, where plain accesses are faster.
+constexpr size_t kStreamFieldBytes = size_t(256) << 20;
+inline bool stream_fields(size_t field_bytes) {
+    if (const char *v = std::getenv("CFD_STREAM_FIELDS")) return std::string(v) != "0";  // A/B override
+    return field_bytes > kStreamFieldBytes;
+}
+
+// Merged half-step launch (k_full_h / k_full_e): cs = interior stretch as
+// scalars, nt = streamed fields; B = batched instances (blockIdx.y).
+template <class R, bool B>
+void launch_full(hipStream_t s, bool h, bool cs, bool nt, dim3 grid, const Dev<R> &d0, const Dev<R> *devs,
+                 const TileGrid &g, const Box &L, const LeanStretch<R> &st)
+{
+    const dim3 blk(TX, TY);
+#define CFD_FULL(K, CS, NT) hipLaunchKernelGGL((K<R, CS, NT, B>), grid, blk, 0, s, d0, devs, g, L, st)
+    if (h) {
+        if (cs) { if (nt) CFD_FULL(k_full_h, true, true); else CFD_FULL(k_full_h, true, false); }
+        else { if (nt) CFD_FULL(k_full_h, false, true); else CFD_FULL(k_full_h, false, false); }
+    } else {
+        if (cs) { if (nt) CFD_FULL(k_full_e, true, true); else CFD_FULL(k_full_e, true, false); }
+        else { if (nt) CFD_FULL(k_full_e, false, true); else CFD_FULL(k_full_e, false, false); }
+    }
+#undef CFD_FULL
+}
+
 class SimBase {
 public:
     virtual ~SimBase() = default;
@@ -244,6 +271,7 @@ public:
         d.je0 = m.pby ? 0 : 1;
         const size_t ncell = (size_t)d.slab * (m.Nz + 1);
         for (int c = 0; c < 6; c++) d.F[c] = alloc<R>(ncell);
+        nt_fields = stream_fields(6 * ncell * sizeof(R));
         // material IDs in device order
         {
             std::vector<uint8_t> id(ncell, 0);
@@ -427,6 +455,7 @@ public:
     bool leanH_cs = false, leanE_cs = false;
     long long *special = nullptr;
     int nspecial = 0;
+    bool nt_fields = false;  // streamed own-cell field traffic (see stream_fields)
 
     static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1, int tw) {
         Box b{};
@@ -533,13 +562,8 @@ public:
             const TileGrid g = tiles();
             const unsigned nb = (unsigned)(g.gx * g.gy * g.gz);
             timer.run(stream, h ? "full_h" : "full_e", [&] {
-                if (h) {
-                    if (leanH_cs) hipLaunchKernelGGL((k_full_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanH_st);
-                    else hipLaunchKernelGGL((k_full_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanH_st);
-                } else {
-                    if (leanE_cs) hipLaunchKernelGGL((k_full_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanE_st);
-                    else hipLaunchKernelGGL((k_full_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, nullptr, g, L, leanE_st);
-                }
+                launch_full<R, false>(stream, h, h ? leanH_cs : leanE_cs, nt_fields, dim3(nb), d, nullptr, g, L,
+                                      h ? leanH_st : leanE_st);
             });
             HIP_OK(hipGetLastError());
             if (!h && nspecial) {
@@ -554,21 +578,28 @@ public:
         if (nl) {
             const int skip5 = m.has_gain ? 1 : 0;
             timer.run(stream, h ? "lean_h" : "lean_e", [&] {
+                const dim3 grid((unsigned)nl), blk(TX, TY);
+#define CFD_LEAN_H(CS, NT) hipLaunchKernelGGL((k_lean_h<R, CS, NT>), grid, blk, 0, stream, d, L, leanH_st)
+#define CFD_LEAN_E(CS, NT) hipLaunchKernelGGL((k_lean_e<R, CS, NT>), grid, blk, 0, stream, d, L, leanE_st, skip5)
                 if (h) {
-                    if (leanH_cs) hipLaunchKernelGGL((k_lean_h<R, true>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanH_st);
-                    else hipLaunchKernelGGL((k_lean_h<R, false>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanH_st);
+                    if (leanH_cs) { if (nt_fields) CFD_LEAN_H(true, true); else CFD_LEAN_H(true, false); }
+                    else { if (nt_fields) CFD_LEAN_H(false, true); else CFD_LEAN_H(false, false); }
                 } else {
-                    if (leanE_cs) hipLaunchKernelGGL((k_lean_e<R, true>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanE_st, skip5);
-                    else hipLaunchKernelGGL((k_lean_e<R, false>), dim3((unsigned)nl), dim3(TX, TY), 0, stream, d, L, leanE_st, skip5);
+                    if (leanE_cs) { if (nt_fields) CFD_LEAN_E(true, true); else CFD_LEAN_E(true, false); }
+                    else { if (nt_fields) CFD_LEAN_E(false, true); else CFD_LEAN_E(false, false); }
                 }
+#undef CFD_LEAN_H
+#undef CFD_LEAN_E
             });
             HIP_OK(hipGetLastError());
         }
         const int nb = h ? shellH_blocks : shellE_blocks;
         if (nb) {
             timer.run(stream, h ? "shell_h" : "shell_e", [&] {
-                if (h) hipLaunchKernelGGL(k_shell_h<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
-                else hipLaunchKernelGGL(k_shell_e<R>, dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
+                if (h && nt_fields) hipLaunchKernelGGL((k_shell_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
+                else if (h) hipLaunchKernelGGL((k_shell_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
+                else if (nt_fields) hipLaunchKernelGGL((k_shell_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
+                else hipLaunchKernelGGL((k_shell_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
             });
             HIP_OK(hipGetLastError());
         }
@@ -837,6 +868,7 @@ public:
     int nports = 0;
     int dft_pending = 0, dft_first = 0;
     size_t own_bytes = 0;
+    bool nt_fields = false;  // by the field bytes of all instances together
 
     Batch(std::vector<Model> &&models, int prec) {
         if (models.empty()) throw Error("empty batch");
@@ -847,6 +879,7 @@ public:
                 sims.back()->member = true;
             }
             check_shape();
+            nt_fields = stream_fields(6 * (size_t)sims[0]->d.slab * (sims[0]->m.Nz + 1) * sizeof(R) * sims.size());
             std::vector<Dev<R>> dv;
             std::vector<SpecialList> sl;
             std::vector<PortDesc<R>> pa;
@@ -920,14 +953,12 @@ public:
             const int n = done;
             if (n >= a.m.nt) throw Error("run finished: all " + std::to_string(a.m.nt) + " steps done");
             timer.run(stream, "full_h", [&] {
-                if (a.leanH_cs) hipLaunchKernelGGL((k_full_h<R, true, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanH, a.leanH_st);
-                else hipLaunchKernelGGL((k_full_h<R, false, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanH, a.leanH_st);
+                launch_full<R, true>(stream, true, a.leanH_cs, nt_fields, dim3(nb, ni), d0, devs, g, a.leanH, a.leanH_st);
             });
             HIP_OK(hipGetLastError());
             tfsf(true, n);
             timer.run(stream, "full_e", [&] {
-                if (a.leanE_cs) hipLaunchKernelGGL((k_full_e<R, true, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanE, a.leanE_st);
-                else hipLaunchKernelGGL((k_full_e<R, false, true>), dim3(nb, ni), dim3(TX, TY), 0, stream, d0, devs, g, a.leanE, a.leanE_st);
+                launch_full<R, true>(stream, false, a.leanE_cs, nt_fields, dim3(nb, ni), d0, devs, g, a.leanE, a.leanE_st);
             });
             HIP_OK(hipGetLastError());
             if (max_special) {

@@ -24,20 +24,23 @@ __device__ __forceinline__ long long cell_index(int px, int py, int i, int j, in
 }
 
 // A half-step reads the updated component at its own cell once and writes it
-// once; no other tile reads it in that pass.  Those loads and stores are
-// non-temporal, so they stream past the caches and leave room for the
-// neighbour values that adjacent tiles re-read (measured on the 256^3 FP32
-// workload: both merged kernels 4-5 % faster, stores alone change nothing;
-// FP64 split form 4.5 % faster).
-template <class T>
+// once; no other tile reads it in that pass.  When the fields do not fit in
+// the Infinity Cache (NT), those loads and stores are non-temporal, so they
+// stream past the caches and leave room for the neighbour values that
+// adjacent tiles re-read (256^3: FP32 merged kernels 4-5 % faster, FP64 step
+// 4.5 % faster).  When they fit, the cache keeps a pass's output for the next
+// pass and plain accesses are faster (128^3: 3-10 %).
+template <bool NT, class T>
 __device__ __forceinline__ T ld_once(const T *p)
 {
-    return __builtin_nontemporal_load(p);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
-template <class T>
+template <bool NT, class T>
 __device__ __forceinline__ void st_once(T *p, T v)
 {
-    __builtin_nontemporal_store(v, p);
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,9 +348,9 @@ __global__ __launch_bounds__(TX *TY) void k_update_h(Dev<R> d, unsigned parts, T
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     h_cell(d, (parts & H_BASIC) != 0, pml ? 7u : 0u, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
            d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz);
-    st_once(&d.F[3][p], hx);
-    st_once(&d.F[4][p], hy);
-    st_once(&d.F[5][p], hz);
+    st_once<false>(&d.F[3][p], hx);
+    st_once<false>(&d.F[4][p], hy);
+    st_once<false>(&d.F[5][p], hz);
 }
 
 template <class R>
@@ -371,9 +374,9 @@ __global__ __launch_bounds__(TX *TY) void k_update_e(Dev<R> d, unsigned parts, T
     R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
     e_cell(d, parts, pml ? 7u : 0u, i, j, k, p, id, d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm], d.F[3][pkm],
            d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
-    st_once(&d.F[0][p], E[0]);
-    st_once(&d.F[1][p], E[1]);
-    st_once(&d.F[2][p], E[2]);
+    st_once<false>(&d.F[0][p], E[0]);
+    st_once<false>(&d.F[1][p], E[1]);
+    st_once<false>(&d.F[2][p], E[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -434,7 +437,7 @@ __device__ __forceinline__ R lean_st(const Dev<R> &d, const LeanStretch<R> &c, i
 }
 
 // interior H: BasicFDTD.cpp:67-81 only
-template <class R, bool CS>
+template <class R, bool CS, bool NT>
 __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<R> c)
 {
     const int t = blockIdx.x;
@@ -447,19 +450,19 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<
     const long long pk1 = p + d.slab;
     const int id = d.id[p];  // first: the material lookup then overlaps the field loads
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
-    R hx = ld_once(&d.F[3][p]), hy = ld_once(&d.F[4][p]), hz = ld_once(&d.F[5][p]);
+    R hx = ld_once<NT>(&d.F[3][p]), hy = ld_once<NT>(&d.F[4][p]), hz = ld_once<NT>(&d.F[5][p]);
     const R aH = d.aH[id], bH = d.bH[id];
     hx = aH * hx - bH * (d.F[2][pj1] - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (d.F[1][pk1] - ey) * lean_st<CS>(d, c, S_cHxy, k);
     hy = aH * hy - bH * (d.F[0][pk1] - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (d.F[2][pi1] - ez) * lean_st<CS>(d, c, S_cHyz, i);
     hz = aH * hz + bH * (d.F[0][pj1] - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (d.F[1][pi1] - ey) * lean_st<CS>(d, c, S_cHzy, i);
-    st_once(&d.F[3][p], hx);
-    st_once(&d.F[4][p], hy);
-    st_once(&d.F[5][p], hz);
+    st_once<NT>(&d.F[3][p], hx);
+    st_once<NT>(&d.F[4][p], hy);
+    st_once<NT>(&d.F[5][p], hz);
 }
 
 // interior E: BasicFDTD.cpp:25-45 for plain cells (ID 4 zeroed; ID 2 and, with a
 // gain medium, ID 5 are the special pass's)
-template <class R, bool CS>
+template <class R, bool CS, bool NT>
 __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<R> c, int skip5)
 {
     const int t = blockIdx.x;
@@ -476,7 +479,7 @@ __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<
     // special pass store their own value back unchanged.
     const int id = d.id[p];
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
-    const R ex = ld_once(&d.F[0][p]), ey = ld_once(&d.F[1][p]), ez = ld_once(&d.F[2][p]);
+    const R ex = ld_once<NT>(&d.F[0][p]), ey = ld_once<NT>(&d.F[1][p]), ez = ld_once<NT>(&d.F[2][p]);
     const R hz_jm = d.F[5][pjm], hy_km = d.F[4][pkm], hx_km = d.F[3][pkm];
     const R hz_im = d.F[5][pim], hy_im = d.F[4][pim], hx_jm = d.F[3][pjm];
     const R aE = d.aE[id], bE = d.bE[id];
@@ -487,13 +490,13 @@ __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<
     e0 = keep ? ex : id == 4 ? (R)0 : e0;
     e1 = keep ? ey : id == 4 ? (R)0 : e1;
     e2 = keep ? ez : id == 4 ? (R)0 : e2;
-    st_once(&d.F[0][p], e0);
-    st_once(&d.F[1][p], e1);
-    st_once(&d.F[2][p], e2);
+    st_once<NT>(&d.F[0][p], e0);
+    st_once<NT>(&d.F[1][p], e1);
+    st_once<NT>(&d.F[2][p], e2);
 }
 
 // CPML slab boxes: full physics, slab tests limited to the box's axes
-template <class R>
+template <class R, bool NT>
 __global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, BoxSet S)
 {
     int i, j, k, bi;
@@ -506,9 +509,9 @@ __global__ __launch_bounds__(TX *TY) void k_shell_h(Dev<R> d, BoxSet S)
     R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     h_cell(d, true, (unsigned)S.b[bi].pml, i, j, k, p, d.F[0][p], d.F[1][p], d.F[2][p], d.F[0][pj1], d.F[2][pj1],
            d.F[0][pk1], d.F[1][pk1], d.F[1][pi1], d.F[2][pi1], hx, hy, hz, id);
-    st_once(&d.F[3][p], hx);
-    st_once(&d.F[4][p], hy);
-    st_once(&d.F[5][p], hz);
+    st_once<NT>(&d.F[3][p], hx);
+    st_once<NT>(&d.F[4][p], hy);
+    st_once<NT>(&d.F[5][p], hz);
 }
 
 template <class R>
@@ -521,14 +524,14 @@ __device__ __forceinline__ void e_full(const Dev<R> &d, int i, int j, int k, uns
     R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
     e_cell(d, E_ALL, pm, i, j, k, p, d.id[p], d.F[3][p], d.F[4][p], d.F[5][p], d.F[3][pjm], d.F[5][pjm],
            d.F[3][pkm], d.F[4][pkm], d.F[4][pim], d.F[5][pim], E);
-    st_once(&d.F[0][p], E[0]);
-    st_once(&d.F[1][p], E[1]);
-    st_once(&d.F[2][p], E[2]);
+    st_once<false>(&d.F[0][p], E[0]);
+    st_once<false>(&d.F[1][p], E[1]);
+    st_once<false>(&d.F[2][p], E[2]);
 }
 
 // CPML shell, E: stencil and psi only.  Drude-Lorentz and gain cells of the
 // shell belong to the special pass, which also runs their psi updates.
-template <class R>
+template <class R, bool NT>
 __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
 {
     int i, j, k, bi;
@@ -538,7 +541,7 @@ __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
     const int id = d.id[p];
-    R E[3] = {ld_once(&d.F[0][p]), ld_once(&d.F[1][p]), ld_once(&d.F[2][p])};
+    R E[3] = {ld_once<NT>(&d.F[0][p]), ld_once<NT>(&d.F[1][p]), ld_once<NT>(&d.F[2][p])};
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
@@ -549,9 +552,9 @@ __global__ __launch_bounds__(TX *TY) void k_shell_e(Dev<R> d, BoxSet S)
            hz_im, E, true);
     const bool sp = special_id(d, id);
     for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
-    st_once(&d.F[0][p], E[0]);
-    st_once(&d.F[1][p], E[1]);
-    st_once(&d.F[2][p], E[2]);
+    st_once<NT>(&d.F[0][p], E[0]);
+    st_once<NT>(&d.F[1][p], E[1]);
+    st_once<NT>(&d.F[2][p], E[2]);
 }
 
 // Batched launches: one launch covers every instance of a batch (gridDim.y
@@ -586,7 +589,7 @@ __device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, int x0, int y
     return m;
 }
 
-template <class R, bool CS, bool B = false>
+template <class R, bool CS, bool NT, bool B = false>
 __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs, TileGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance<R, B>(d0, devs);
@@ -605,7 +608,7 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
     const R ex_j1 = d.F[0][pj1], ez_j1 = d.F[2][pj1], ex_k1 = d.F[0][pk1];
     const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
-    R hx = ld_once(&d.F[3][p]), hy = ld_once(&d.F[4][p]), hz = ld_once(&d.F[5][p]);
+    R hx = ld_once<NT>(&d.F[3][p]), hy = ld_once<NT>(&d.F[4][p]), hz = ld_once<NT>(&d.F[5][p]);
     if (tile_inside(L, x0, y0, k)) {
         const R aH = d.aH[id], bH = d.bH[id];
         hx = aH * hx - bH * (ez_j1 - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (ey_k1 - ey) * lean_st<CS>(d, c, S_cHxy, k);
@@ -615,12 +618,12 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
         h_cell(d, true, d.has_pml ? tile_pml_mask(d, x0, y0, k, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
                ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz, id);
     }
-    st_once(&d.F[3][p], hx);
-    st_once(&d.F[4][p], hy);
-    st_once(&d.F[5][p], hz);
+    st_once<NT>(&d.F[3][p], hx);
+    st_once<NT>(&d.F[4][p], hy);
+    st_once<NT>(&d.F[5][p], hz);
 }
 
-template <class R, bool CS, bool B = false>
+template <class R, bool CS, bool NT, bool B = false>
 __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs, TileGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance<R, B>(d0, devs);
@@ -634,7 +637,7 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
     const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
     const long long pkm = p - d.slab;
     const int id = d.id[p];
-    R E[3] = {ld_once(&d.F[0][p]), ld_once(&d.F[1][p]), ld_once(&d.F[2][p])};
+    R E[3] = {ld_once<NT>(&d.F[0][p]), ld_once<NT>(&d.F[1][p]), ld_once<NT>(&d.F[2][p])};
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
@@ -655,9 +658,9 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
         const bool sp = special_id(d, id);
         for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
     }
-    st_once(&d.F[0][p], E[0]);
-    st_once(&d.F[1][p], E[1]);
-    st_once(&d.F[2][p], E[2]);
+    st_once<NT>(&d.F[0][p], E[0]);
+    st_once<NT>(&d.F[1][p], E[1]);
+    st_once<NT>(&d.F[2][p], E[2]);
 }
 
 // Drude-Lorentz and gain cells, all of them: device cell index, with bit
