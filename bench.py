@@ -221,7 +221,7 @@ def batched_c4(steps=200, warmup=10, n=512):
             "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()}}
 
 
-def serving_c5(nreq=120, precision=32, quantum=256):
+def serving_c5(nreq=120, precision=32, quantum=256, policy=0):
     """The first ``nreq`` requests of the serving workload's stream (Poisson
     20/s, log-uniform 32-320 edges, 500-8000 steps, sphere / Drude-Lorentz /
     gain mix), submitted at their arrival times to the server: latency from
@@ -237,7 +237,7 @@ def serving_c5(nreq=120, precision=32, quantum=256):
         os.makedirs(d)
         cases.append((t, configs.c5(d, p)))
     work = sum(p["N"][0] * p["N"][1] * p["N"][2] * p["steps"] for _, p in rq)
-    srv = eng.Server(quantum=quantum)
+    srv = eng.Server(quantum=quantum, policy=policy)
     try:
         t0 = time.perf_counter()
         ids = []
@@ -267,7 +267,8 @@ def serving_c5(nreq=120, precision=32, quantum=256):
             "value": round(work / span / 1e6, 2), "unit": "Mcell-updates/s (served)",
             "arrival_span_s": round(rq[-1][0], 2), "makespan_s": round(span, 2),
             "demand_mcells": round(work / rq[-1][0] / 1e6, 1), "peak_active": stats["peak_active"],
-            "peak_device_gb": round(stats["peak_bytes"] / 1e9, 2), "quantum": quantum}
+            "peak_device_gb": round(stats["peak_bytes"] / 1e9, 2), "quantum": quantum,
+            "policy": "shortest work first" if policy == eng.POLICY_SHORTEST else "first come, first served"}
 
 
 def other_configs(steps=400, warmup=20):
@@ -429,12 +430,15 @@ def main():
             log("bench: batched c4 failed: %s" % ex)
     if not args.no_serve:
         log("bench: serving the first %d c5 requests" % args.serve_requests)
-        try:
-            out["serving"] = serving_c5(args.serve_requests)
-            log("bench: serving p50 %.2f s, p99 %.2f s" % (out["serving"]["p50_s"], out["serving"]["p99_s"]))
-        except Exception as ex:
-            out["serving"] = {"error": str(ex)}
-            log("bench: serving failed: %s" % ex)
+        # the server's default admission (first come, first served), then
+        # shortest work first on the same stream
+        for key, pol in (("serving", eng.POLICY_FIFO), ("serving_shortest_first", eng.POLICY_SHORTEST)):
+            try:
+                out[key] = serving_c5(args.serve_requests, policy=pol)
+                log("bench: %s p50 %.2f s, p99 %.2f s" % (key, out[key]["p50_s"], out[key]["p99_s"]))
+            except Exception as ex:
+                out[key] = {"error": str(ex)}
+                log("bench: %s failed: %s" % (key, ex))
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()

@@ -75,6 +75,7 @@ struct Request {
     std::unique_ptr<ConfigCopy> cfg;
     cfd_model *model = nullptr;  // built ahead of admission
     size_t estimate = 0;
+    double work = 0;             // cells x steps, once the model is built
     cfd_sim *sim = nullptr;
     size_t bytes = 0;
     int nt = 0, done = 0;
@@ -153,33 +154,60 @@ struct cfd_server {
         cv_done.notify_all();
     }
 
-    // host thread: start the queue head if it fits; false when it must wait
+    // host thread: a queued request's model, its device-memory estimate and
+    // work; a request whose input fails to build fails alone (false)
+    bool ensure_model(Request &r) {
+        if (r.model) return true;
+        if (cfd_model_build(r.cfg->get(), &r.model) < 0) {
+            const std::string err = cfd_model_last_error();
+            r.model = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                queue.erase(std::find(queue.begin(), queue.end(), r.id));
+            }
+            finish(r, false, err);
+            return false;
+        }
+        int n[4] = {0, 0, 0, 0};
+        cfd_model_dims(r.model, n, nullptr);
+        r.estimate = estimate(r.cfg->cfg, n);
+        r.work = (double)n[0] * n[1] * n[2] * std::max(n[3], 1);
+        return true;
+    }
+
+    // host thread: start the next request by the policy (the queue head, or
+    // the queued request with the least work) if it fits; false when it must
+    // wait
     bool admit_head() {
         Request *r = nullptr;
+        std::vector<Request *> queued;
         {
             std::lock_guard<std::mutex> lk(mu);
             if (queue.empty() || (int)active.size() >= opts.max_active) return false;
-            r = reqs[queue.front()].get();
+            if (opts.policy == CFD_POLICY_SHORTEST)
+                for (long long id : queue) queued.push_back(reqs[id].get());
+            else
+                r = reqs[queue.front()].get();
         }
-        if (!r->model) {
-            if (cfd_model_build(r->cfg->get(), &r->model) < 0) {
-                const std::string err = cfd_model_last_error();
-                r->model = nullptr;
-                { std::lock_guard<std::mutex> lk(mu); queue.pop_front(); }
-                finish(*r, false, err);
-                return true;
+        if (r) {
+            if (!ensure_model(*r)) return true;
+        } else {
+            for (Request *x : queued)
+                if (!ensure_model(*x)) return true;
+            std::lock_guard<std::mutex> lk(mu);
+            for (long long id : queue) {  // requests submitted meanwhile wait for the next call
+                Request *x = reqs[id].get();
+                if (x->model && (!r || x->work < r->work)) r = x;
             }
-            int n[4] = {0, 0, 0, 0};
-            cfd_model_dims(r->model, n, nullptr);
-            r->estimate = estimate(r->cfg->cfg, n);
+            if (!r) return true;
         }
         {
             std::lock_guard<std::mutex> lk(mu);
             if (!active.empty() && active_bytes + r->estimate > opts.hbm_budget) {
-                budget_blocked = true;  // first come, first served: wait for a release
+                budget_blocked = true;  // wait for a release
                 return false;
             }
-            queue.pop_front();
+            queue.erase(std::find(queue.begin(), queue.end(), r->id));
             r->state = CFD_REQ_RUNNING;
             r->t_start = now();
         }
@@ -309,6 +337,7 @@ int cfd_server_create(const cfd_server_opts *o, cfd_server **out)
     // 256-step quanta are two replayed 128-step graph blocks
     s->opts.max_active = o && o->max_active > 0 ? o->max_active : 4;
     s->opts.quantum = o && o->quantum > 0 ? o->quantum : 256;
+    s->opts.policy = o && o->policy == CFD_POLICY_SHORTEST ? CFD_POLICY_SHORTEST : CFD_POLICY_FIFO;
     cfd_server *p = s.get();
     p->stepper = std::thread([p] { p->step_loop(); });
     p->host = std::thread([p] { p->host_loop(); });

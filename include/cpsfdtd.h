@@ -151,10 +151,13 @@ int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *lau
 #define CFD_REQ_FAILED  3
 #define CFD_RESULT_INTENSITY 0   /* [steps] sum over the port of |E|^2 (Ports.cpp:512-522) */
 #define CFD_RESULT_DFT       1   /* [points][n_freq][6][re,im] */
+#define CFD_POLICY_FIFO     0   /* admit in arrival order */
+#define CFD_POLICY_SHORTEST 1   /* admit the queued request with the least work (cells x steps) */
 typedef struct {
     size_t hbm_budget;   /* bytes of device memory for running requests (0: 200 GiB) */
     int max_active;      /* requests running at once (0: 4) */
     int quantum;         /* steps per scheduling slice (0: 256) */
+    int policy;          /* CFD_POLICY_* (0: FIFO) */
 } cfd_server_opts;
 typedef struct cfd_server cfd_server;
 int cfd_server_create(const cfd_server_opts *opts, cfd_server **out);

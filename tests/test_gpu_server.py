@@ -98,6 +98,29 @@ def test_server_watchdog_fails_a_diverging_request(tmp_path):
     srv.close()
 
 
+@pytest.mark.parametrize("policy", [eng.POLICY_FIFO, eng.POLICY_SHORTEST], ids=["fifo", "shortest"])
+def test_server_admission_policy(tmp_path, policy):
+    """One request at a time: while a first request holds the slot, a large
+    and then a small request queue; first come, first served starts the
+    large one next, shortest work first (cells x steps) the small one."""
+    import time
+    srv = eng.Server(max_active=1, quantum=64, policy=policy)
+    first = make(tmp_path, 0, {"N": (96, 96, 96), "steps": 3000, "kind": "sphere"})
+    large = make(tmp_path, 1, {"N": (60, 60, 60), "steps": 600, "kind": "sphere"})
+    small = make(tmp_path, 2, {"N": (32, 32, 32), "steps": 100, "kind": "sphere"})
+    a = srv.submit(first, 32)
+    while srv.poll(a)[0] == eng.REQ_QUEUED:
+        time.sleep(0.005)
+    b = srv.submit(large, 32)
+    c = srv.submit(small, 32)
+    assert srv.poll(a)[0] == eng.REQ_RUNNING  # both queued behind it
+    for rid in (a, b, c):
+        assert srv.wait(rid, 120.0) == eng.REQ_DONE, srv.error(rid)
+    tb, tc = srv.poll(b)[1][1], srv.poll(c)[1][1]
+    assert (tc < tb) if policy == eng.POLICY_SHORTEST else (tb < tc)
+    srv.close()
+
+
 def test_engine_nonfinite_count(tmp_path):
     case = make(tmp_path, 0, SMALL[0][0])
     e = eng.Engine(case, 64)
