@@ -92,6 +92,7 @@ struct Request {
 // budget), reads back finished requests and releases them, so host work never
 // holds up the GPU queue.
 struct cfd_server {
+    static constexpr int kShortestWindow = 64;  // queued requests a shortest-first choice looks at
     cfd_server_opts opts{};
     std::mutex mu;
     std::condition_variable cv_host, cv_step, cv_done;
@@ -184,9 +185,14 @@ struct cfd_server {
         {
             std::lock_guard<std::mutex> lk(mu);
             if (queue.empty() || (int)active.size() >= opts.max_active) return false;
-            if (opts.policy == CFD_POLICY_SHORTEST)
-                for (long long id : queue) queued.push_back(reqs[id].get());
-            else
+            if (opts.policy == CFD_POLICY_SHORTEST) {
+                // the least work among the oldest kShortestWindow requests: only
+                // their models are held, whatever the queue's length
+                for (long long id : queue) {
+                    if ((int)queued.size() == kShortestWindow) break;
+                    queued.push_back(reqs[id].get());
+                }
+            } else
                 r = reqs[queue.front()].get();
         }
         if (r) {
@@ -195,10 +201,8 @@ struct cfd_server {
             for (Request *x : queued)
                 if (!ensure_model(*x)) return true;
             std::lock_guard<std::mutex> lk(mu);
-            for (long long id : queue) {  // requests submitted meanwhile wait for the next call
-                Request *x = reqs[id].get();
-                if (x->model && (!r || x->work < r->work)) r = x;
-            }
+            for (Request *x : queued)  // all still queued: only this thread removes requests
+                if (!r || x->work < r->work) r = x;
             if (!r) return true;
         }
         {
