@@ -177,3 +177,60 @@ def test_graph_replay_equals_eager_fp32(tmp_path, monkeypatch):
         e.close()
     for a, b in zip(runs[0][0] + runs[0][1], runs[1][0] + runs[1][1]):
         assert np.array_equal(a, b)
+
+
+def _odd_case(directory, n, npml, dl):
+    """A grid of ragged shape: a dielectric block, an optional Drude-Lorentz
+    block, a soft Ez source and CPML of the given thickness per axis."""
+    from cpsfdtd_amd.case import CaseWriter, Shape
+    d = 10e-9
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (1, 1, 0, 0)])
+    shapes = [Shape(1, "cuboid", x0=d, y0=d, z0=d, a=(n[0] // 2) * d, b=(n[1] // 2) * d, c=(n[2] // 2) * d)]
+    if dl:
+        shapes.append(Shape(2, "cuboid", x0=(n[0] // 2) * d, y0=0.0, z0=0.0, a=(n[0] + 1) * d, b=(n[1] + 1) * d,
+                            c=(n[2] // 3 + 1) * d))
+    w.grid(n, (d, d, d), shapes)
+    if any(npml):
+        w.cpml(((npml[0], npml[0]), (npml[1], npml[1]), (npml[2], npml[2])), m=3.0, malp=1.0, kapmax=5.0,
+               sig_ratio=1.0, alpmax=0.05)
+    w.source(500e-9, 5e-15, 10e-15)
+    w.point_source(n[0] // 2, n[1] // 2, n[2] // 2, comp=2, amp=1.0)
+    if dl:
+        w.drude_lorentz(3.7, [(1.37e16, 2.7e13)], [(4.2e15, 1e15, 1.0)])
+    return w.run(30, flags=FIXED)
+
+
+@pytest.mark.parametrize("form", ["merged", "split", "grid"])
+@pytest.mark.parametrize("n,npml,dl", [
+    ((2, 2, 2), (0, 0, 0), False),        # the smallest grid the model accepts
+    ((3, 70, 5), (0, 4, 0), False),       # one ragged x-tile, y spans 18 tile rows
+    ((65, 3, 4), (4, 0, 0), True),        # x just past one tile width, CPML in x
+    ((130, 9, 7), (5, 2, 2), True),       # three x-tiles, the last with 2 columns; thin CPML on every axis
+], ids=["2x2x2", "3x70x5", "65x3x4", "130x9x7"])
+def test_ragged_shapes_f64_bit_exact(tmp_path, monkeypatch, form, n, npml, dl):
+    """Tile edges in every form: partial x-tiles and tile rows, axes shorter
+    than a tile, slabs of a few cells, Drude-Lorentz cells in a CPML slab."""
+    monkeypatch.setenv("CFD_UPDATE_KERNEL", form)
+    case = _odd_case(str(tmp_path), n, npml, dl)
+    e, o = run_both(case, 30)
+    assert_fields_equal(e, o, "%s %s" % (form, n))
+
+
+def test_empty_case_steps_and_stays_zero(tmp_path):
+    """A grid with nothing in it (no sources, TF/SF, CPML or ports) steps in
+    both precisions, with and without graph replay, and stays exactly zero."""
+    from cpsfdtd_amd.case import CaseWriter
+    w = CaseWriter(str(tmp_path))
+    w.materials([(1, 1, 0, 0)])
+    w.grid((33, 17, 9), (10e-9, 10e-9, 10e-9), [])
+    case = w.run(300)
+    for prec in (32, 64):
+        e = eng.Engine(case, prec)
+        e.step(7)      # eager
+        e.step(256)    # includes graph-replayed blocks
+        e.sync()
+        for c in range(6):
+            assert not np.any(e.field(c)), (prec, c)
+        assert e.nonfinite() == 0
+        e.close()
