@@ -24,6 +24,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -58,6 +60,35 @@ def pass_geometry(N, npml, periodic=0):
             slab += (n1 + n2) * plane
         out["e" if ef else "h"] = {"lean_box": lean, "lean": nlean, "shell": total - nlean, "slab": slab}
     return out
+
+
+def workload_geometry(model, periodic=0):
+    """pass_geometry of a workload's host model, plus the special-pass cells
+    (Drude-Lorentz and gain) inside and outside the E interior box."""
+    N = model.N
+    npml = [[len(model.table("pml_Hb%s%d" % (ax, e + 1))) for e in range(2)] for ax in "xyz"]
+    geo = pass_geometry(N, npml, periodic)
+    lb = geo["e"]["lean_box"]
+    inside = total = 0
+    for gain in (False, True):
+        f = np.asarray(model.cells(gain), np.int64)
+        if len(f) == 0:
+            continue
+        i, j, k = f // ((N[1] + 1) * (N[2] + 1)), (f // (N[2] + 1)) % (N[1] + 1), f % (N[2] + 1)
+        inside += int(((lb[0] <= i) & (i < lb[1]) & (lb[2] <= j) & (j < lb[3]) & (lb[4] <= k) & (k < lb[5])).sum())
+        total += len(f)
+    geo["dl_interior"], geo["dl_shell"] = inside, total - inside
+    return geo
+
+
+def update_gbs(geo, S, kt):
+    """Algorithmic GB/s of the H and E update kernels together (whichever
+    form ran), from per-kernel times over the same steps."""
+    ab = algorithmic_bytes(geo, S, 0)
+    names = [k for k in ("full_h", "full_e", "lean_h", "shell_h", "lean_e", "shell_e") if k in kt]
+    t = sum(kt[k][0] for k in names) / 1e3
+    nbytes = sum(ab[k] * kt[k][1] for k in names)
+    return round(nbytes / t / 1e9, 1) if t > 0 else None
 
 
 def algorithmic_bytes(geo, S, nd_vars):
@@ -280,8 +311,11 @@ def other_configs(steps=400, warmup=20):
     from cpsfdtd_amd import configs
     out = {}
     for name, make in (("c1", configs.c1), ("c3", configs.c3)):
-        case = make(tempfile.mkdtemp(prefix="bench_%s_" % name), nt=warmup + steps + 1)
-        N = eng.Model(case).N
+        case = make(tempfile.mkdtemp(prefix="bench_%s_" % name), nt=warmup + steps + 41)
+        model = eng.Model(case)
+        N = model.N
+        geo = workload_geometry(model, case.periodic)
+        del model
         cells = N[0] * N[1] * N[2]
         row = {"cells": cells, "steps": steps}
         for prec in (32, 64):
@@ -292,8 +326,13 @@ def other_configs(steps=400, warmup=20):
             e.step(steps)
             e.sync()
             wall = time.perf_counter() - t0
+            e.set_timing(True)
+            e.step(40)
+            e.sync()
+            kt = e.kernel_times()
             e.close()
             row["f%d" % prec] = round(cells * steps / wall / 1e6, 1)
+            row["f%d_update_gbs" % prec] = update_gbs(geo, prec // 8, kt)
         out[name] = row
     return out
 
@@ -339,18 +378,8 @@ def main():
     model = eng.Model(case)
     N = model.N
     cells = N[0] * N[1] * N[2]
-    npml = [[len(model.table("pml_Hb%s%d" % (ax, e + 1))) for e in range(2)] for ax in "xyz"]
     nd_vars = 1 + 2 * 2 + 1  # 1 Drude + 2 Lorentz poles: JD, JL, JL1, E1
-    geo = pass_geometry(N, npml)
-    lb = geo["e"]["lean_box"]
-    dl_in = 0
-    cells_dl = model.cells(False)
-    for f in cells_dl:
-        f = int(f)
-        i, j, k = f // ((N[1] + 1) * (N[2] + 1)), (f // (N[2] + 1)) % (N[1] + 1), f % (N[2] + 1)
-        if lb[0] <= i < lb[1] and lb[2] <= j < lb[3] and lb[4] <= k < lb[5]:
-            dl_in += 1
-    geo["dl_interior"], geo["dl_shell"] = dl_in, len(cells_dl) - dl_in
+    geo = workload_geometry(model)
     del model
 
     res = {}
