@@ -144,18 +144,20 @@ def run_engine(case, precision, steps, warmup, timing_steps):
     kt = e.kernel_times()
     e.set_timing(False)
     nbytes = e.device_bytes()
-    results = [(e.port_intensity(q), e.port_dft(q)) for q in range(len(case.ports))]
+    # per port: |E|^2 series, DFT, and (with DFT frequencies) the spectral
+    # energy through the plane (the workload's ports are z-normal planes)
+    results = [(e.port_intensity(q), e.port_dft(q), e.port_flux(q, 2) if case.freqs else None)
+               for q in range(len(case.ports))]
     e.close()
     return wall, kt, nbytes, results
 
 
-def spectrum_peak(freqs, dft):
-    """Peak frequency and Q = f0 / FWHM of a port's power spectrum
-    sum over points and E components of |E(f)|^2 (half-maximum crossings by
-    linear interpolation; None where the peak is not bracketed)."""
-    import numpy as np
+def spectrum_peak(freqs, spectrum):
+    """Peak frequency and Q = f0 / FWHM of a spectrum (the transmitted
+    port's spectral energy; half-maximum crossings by linear interpolation;
+    None where the peak is not bracketed)."""
     f = np.asarray(freqs)
-    p = (np.abs(dft[:, :, :3]) ** 2).sum(axis=(0, 2))  # dft: complex [point][freq][6]
+    p = np.asarray(spectrum, np.float64)
     order = np.argsort(f)
     f, p = f[order], p[order]
     k = int(np.argmax(p))
@@ -174,8 +176,8 @@ def spectrum_peak(freqs, dft):
 
 def quality(res32, res64, freqs):
     """FP32 against FP64 over the same run: relative L2 of every port's |E|^2
-    time series and DFT, and the transmitted port's spectral peak and Q."""
-    import numpy as np
+    time series, DFT and flux spectrum, and the transmitted port's flux
+    spectrum peak and Q."""
 
     def rel(a, b):
         den = np.linalg.norm(np.ravel(b))
@@ -183,8 +185,9 @@ def quality(res32, res64, freqs):
     out = {"port_series_rel_l2": [round(rel(a[0], b[0]), 9) for a, b in zip(res32, res64)],
            "port_dft_rel_l2": [round(rel(a[1], b[1]), 9) for a, b in zip(res32, res64)]}
     if freqs and res64:
-        f32, q32 = spectrum_peak(freqs, res32[-1][1])
-        f64, q64 = spectrum_peak(freqs, res64[-1][1])
+        out["port_flux_rel_l2"] = [round(rel(a[2], b[2]), 9) for a, b in zip(res32, res64)]
+        f32, q32 = spectrum_peak(freqs, res32[-1][2])
+        f64, q64 = spectrum_peak(freqs, res64[-1][2])
         out.update({"peak_thz_f32": round(f32 / 1e12, 4), "peak_thz_f64": round(f64 / 1e12, 4),
                     "q_f32": q32 and round(q32, 4), "q_f64": q64 and round(q64, 4)})
     return out

@@ -108,6 +108,8 @@ SIGNATURES = {
     "cfd_server_release": ([_vp, C.c_longlong], C.c_int),
     "cfd_server_stats": ([_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_size_t), _ip, _ip], C.c_int),
     "cfd_server_last_error": ([], C.c_char_p),
+    "cfd_port_flux": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
+    "cfd_dft_flux": ([_dp, C.c_int, C.c_int, C.c_int, C.c_double, _dp], C.c_int),
 }
 
 _lib = None
@@ -307,6 +309,13 @@ class Engine:
         _check(library().cfd_port_dft(self.h, q, a.ctypes.data_as(_dp)))
         return a[..., 0] + 1j * a[..., 1]
 
+    def port_flux(self, q, axis):
+        """Spectral energy through plane port q (normal ``axis``, 0 x 1 y 2 z),
+        J/Hz per DFT frequency (cfd_port_flux)."""
+        a = np.zeros(self.nf, np.float64)
+        _check(library().cfd_port_flux(self.h, q, axis, a.ctypes.data_as(_dp)))
+        return a
+
     def port_intensity(self, q):
         a = np.zeros(self.steps_done, np.float64)
         _check(library().cfd_port_intensity(self.h, q, a.ctypes.data_as(_dp)))
@@ -394,6 +403,17 @@ class Batch:
 
     def kernel_times(self):
         return _kernel_times(library().cfd_batch_kernel_times, self.h)
+
+
+def dft_flux(dft, axis, scale):
+    """cfd_dft_flux on complex DFT accumulators [points][freqs][6] held on the
+    host (a served request's results): scale * Re sum (E_a H_b* - E_b H_a*)."""
+    dft = np.asarray(dft)
+    a = np.ascontiguousarray(np.stack([dft.real, dft.imag], axis=-1), np.float64)
+    out = np.zeros(dft.shape[1], np.float64)
+    if library().cfd_dft_flux(a.ctypes.data_as(_dp), dft.shape[0], dft.shape[1], axis, scale, out.ctypes.data_as(_dp)) < 0:
+        raise EngineError("cfd_dft_flux: bad arguments")
+    return out
 
 
 class _ServerOpts(C.Structure):

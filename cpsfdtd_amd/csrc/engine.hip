@@ -1227,6 +1227,25 @@ int cfd_port_gather(cfd_sim *s, int port, double *host)
 }
 int cfd_port_dft(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_dft(port, host); }); }
 int cfd_port_intensity(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_intensity(port, host); }); }
+int cfd_port_flux(cfd_sim *s, int port, int axis, double *host)
+{
+    NEED(s);
+    if (port < 0 || port >= (int)s->impl->m.ports.size() || axis < 0 || axis > 2 || !host) {
+        cfd::g_error = "no such port, bad axis or null buffer";
+        return -1;
+    }
+    return guard([&] {
+        const cfd::Model &m = s->impl->m;
+        const int npts = m.ports[port].npts(), nf = (int)m.freqs.size();
+        if (!nf) throw cfd::Error("the run has no DFT frequencies");
+        std::vector<double> dft((size_t)npts * nf * 12);
+        s->impl->port_dft(port, dft.data());
+        const double d[3] = {m.dx, m.dy, m.dz};
+        const double h = m.dft_stride * m.dt;  // sample spacing of the running DFT
+        if (cfd_dft_flux(dft.data(), npts, nf, axis, 2.0 * h * h * d[(axis + 1) % 3] * d[(axis + 2) % 3], host) < 0)
+            throw cfd::Error("flux: bad arguments");
+    });
+}
 int cfd_gain_read(cfd_sim *s, double *host) { NEED(s); return guard([&] { s->impl->gain_read(host); }); }
 
 int cfd_set_timing(cfd_sim *s, int on)

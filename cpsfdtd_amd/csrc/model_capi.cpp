@@ -125,4 +125,26 @@ int cfd_model_cells(cfd_model *p, int gain, long long *flat)
     return (int)v.size();
 }
 
+int cfd_dft_flux(const double *dft, int npts, int nf, int axis, double scale, double *out)
+{
+    if (!dft || !out || npts < 0 || nf < 0 || axis < 0 || axis > 2) {
+        g_model_error = "cfd_dft_flux: bad argument";
+        return -1;
+    }
+    // tangential components (a, b) with (a, b, axis) right-handed: S_axis = E_a H_b - E_b H_a
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    for (int f = 0; f < nf; f++) {
+        double acc = 0.0;
+        for (int p = 0; p < npts; p++) {
+            const double *v = dft + ((size_t)p * nf + f) * 12;  // [6][re, im]
+            const double ear = v[2 * a], eai = v[2 * a + 1], ebr = v[2 * b], ebi = v[2 * b + 1];
+            const double har = v[6 + 2 * a], hai = v[6 + 2 * a + 1], hbr = v[6 + 2 * b], hbi = v[6 + 2 * b + 1];
+            // Re(E_a conj(H_b)) - Re(E_b conj(H_a))
+            acc += (ear * hbr + eai * hbi) - (ebr * har + ebi * hai);
+        }
+        out[f] = scale * acc;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -116,6 +116,16 @@ int cfd_port_gather(cfd_sim *s, int port, double *host);        /* [npts][6] cur
                                                                    read back (port_point[ii].Ex ... .Hz) */
 int cfd_port_dft(cfd_sim *s, int port, double *host);           /* [npts][n_freq][6][2] */
 int cfd_port_intensity(cfd_sim *s, int port, double *host);     /* [steps] sum of |E|^2 */
+/* Spectral energy through a plane port, J/Hz, one value per DFT frequency:
+ * 2 (stride dt)^2 Re sum_points (E_a H_b* - E_b H_a*) dA, (a, b, axis) right-
+ * handed, dA the cell face normal to axis (0 x, 1 y, 2 z).  No reference
+ * counterpart (Ports.cpp exports |E|^2 only); E and H at the point's own
+ * indices (half a cell apart along the normal). */
+int cfd_port_flux(cfd_sim *s, int port, int axis, double *host);
+/* The same from DFT accumulators already on the host ([npts][nf][6][2], e.g.
+ * a served request's CFD_RESULT_DFT): out[f] = scale Re sum_p (E_a H_b* -
+ * E_b H_a*); scale = 2 (stride dt)^2 dA gives J/Hz.  No device needed. */
+int cfd_dft_flux(const double *dft, int npts, int nf, int axis, double scale, double *out);
 int cfd_gain_read(cfd_sim *s, double *host);                    /* [cells][4] N0..N3 */
 
 /* Batches: n simulations of one shape (grid, CPML, periodic axes, TF/SF box

@@ -334,3 +334,50 @@ def test_tfsf_vacuum_leakage(oracle_lib, tmp_path, theta, phi, psi):
     assert fixed < 1e-3
     assert corner > 1e-2
     assert ref > corner
+
+
+def test_port_flux_is_the_plane_wave_poynting_spectrum(oracle_lib, tmp_path):
+    """cfd_dft_flux on a plane port's running DFT: (1) equals the same sum
+    computed from numpy transforms of the recorded samples; (2) for a plane
+    wave in vacuum (TF/SF, normal incidence) it is the incident spectral
+    energy 2 |E_inc(f)|^2 / eta0 x port area at two planes inside the box
+    within 1 %; (3) a plane in the scattered-field region sees < 1e-6 of it.
+    A swapped tangential pair, a dropped conjugate or a wrong dt/area scale
+    fails (1) or (2)."""
+    from cpsfdtd_amd.case import dlambda_for_fwhm
+    import cpsfdtd_amd as eng
+    n, npml, box, d, nt = 32, 6, 10, 10e-9, 1000
+    freqs = (5.0e14, 6.0e14, 7.0e14)
+    w = CaseWriter(str(tmp_path))
+    w.materials([(1, 1, 0, 0)])
+    w.grid((n, n, n), (d, d, d), [])
+    w.cpml(npml, m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.tfsf(500e-9, dlambda_for_fwhm(500e-9, 5e-15), 1e-3, 0.0, n * d / 2, n * d / 2, 0.0, 0.0, 0.0, 1.0, 0.0,
+           ((box, box), (box, box), (box, box)), nt)
+    w.source(500e-9, 5e-15, 10e-15)
+    for q, k in enumerate((13, 18, 8)):  # two planes inside the box, one in the scattered field
+        w.plane_port_cells("p%d.txt" % q, "z", k, 12, 20, 12, 20, (d, d, d))
+    case = w.run(nt, freqs=freqs, dft_stride=1, record_ports=True, flags=FIXED)
+    o = oracle_lib.OracleSim(case)
+    o.step(nt)
+    dt = o.dt
+    scale = 2 * dt * dt * d * d
+    fl = [eng.dft_flux(o.port_dft(q), 2, scale) for q in range(3)]
+    # (1) the same sum from the recorded samples
+    x = o.port_samples(0)
+    t = np.arange(nt)
+    for fi, f in enumerate(freqs):
+        E = [np.sum(x[:, :, c] * np.exp(-2j * np.pi * f * (t + 1.0) * dt)[:, None], axis=0) for c in range(2)]
+        H = [np.sum(x[:, :, 3 + c] * np.exp(-2j * np.pi * f * (t + 0.5) * dt)[:, None], axis=0) for c in range(2)]
+        want = scale * np.real(np.sum(E[0] * np.conj(H[1]) - E[1] * np.conj(H[0])))
+        assert abs(fl[0][fi] - want) <= 1e-12 * abs(want)
+    # (2) the incident wave's spectral energy through the port's area
+    einc = eng.Model(case).table("Einc")[:nt]
+    eta0 = math.sqrt(MU0 / EPS0)
+    npts = len(o.port_points(0))
+    inc = np.array([dt * abs(np.sum(einc * np.exp(-2j * np.pi * f * t * dt))) for f in freqs])
+    expect = 2 * inc ** 2 / eta0 * npts * d * d
+    for q in (0, 1):
+        assert np.all(np.abs(fl[q] / expect - 1) < 1e-2), (q, fl[q] / expect)
+    # (3) the scattered-field region is dark
+    assert np.all(np.abs(fl[2]) < 1e-6 * expect)

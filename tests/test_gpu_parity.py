@@ -116,6 +116,10 @@ def test_port_dft_f64(tmp_path):
     e, o = run_both(case, 60)
     for q in range(2):
         assert np.array_equal(e.port_dft(q), o.port_dft(q))
+    # the plane port's flux spectrum: the device run's, the same sum on the
+    # oracle's accumulators (stride 3, 10 nm cells)
+    scale = 2 * (3 * o.dt) ** 2 * 10e-9 * 10e-9
+    assert np.array_equal(e.port_flux(0, 2), eng.dft_flux(o.port_dft(0), 2, scale))
 
 
 def test_fp32_close_to_fp64_oracle(tmp_path):
