@@ -566,7 +566,25 @@ private:
 
 class PlainPort : public detail::PortBase {
 public:
-    PlainPort(Structure &sS, const char *fileName) : PortBase(sS, fileName, CFD_PORT_PLANE) {}
+    PlainPort(Structure &sS, const char *fileName) : PortBase(sS, fileName, CFD_PORT_PLANE)
+    {
+        // Ports.cpp:211-283: normal, position, w1 w2, h1 h2; the first
+        // in-plane index runs over (int)(w1/dw)..(int)(w2/dw), dw = dy for an
+        // x-normal port, else dx
+        std::ifstream in(fileName);
+        std::string line;
+        char normal = 0;
+        double position = 0, w1 = 0, w2 = 0;
+        std::getline(in, line);
+        std::istringstream(line) >> normal;
+        std::getline(in, line);
+        std::istringstream(line) >> position;
+        std::getline(in, line);
+        std::istringstream(line) >> w1 >> w2;
+        const double dw = normal == 'x' ? sS.dy : sS.dx;
+        if (!(dw > 0)) throw Error("PlainPort: construct it after Structure::generate_mesh");
+        nw = (int)(w2 / dw) - (int)(w1 / dw) + 1;
+    }
     void Export_Port(const char *fileName)  // Ports.cpp:355-362
     {
         std::ofstream out(fileName);
@@ -575,38 +593,32 @@ public:
     void Export_Ex_onPort(const char *f) { file_rows(f, [](const Port_Points &q) { return q.Ex; }); }
     void Export_Ey_onPort(const char *f) { file_rows(f, [](const Port_Points &q) { return q.Ey; }); }
     void Export_Ez_onPort(const char *f) { file_rows(f, [](const Port_Points &q) { return q.Ez; }); }
-    void Export_EIntensity_onPort(const char *f) { file_rows(f, intensity); }
+    void Export_EIntensity_onPort(const char *f) { file_line(f, intensity); }
     void Export_Ex_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ex; }); }
     void Export_Ey_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ey; }); }
     void Export_Ez_onPort(std::ofstream &o) { step_row(o, [](const Port_Points &q) { return q.Ez; }); }
     void Export_EIntensity_onPort(std::ofstream &o) { step_row(o, intensity); }
     using PortBase::Export_E_onPort;
 
-    // points along the port's first in-plane axis (Ports.cpp:232-295 Nw)
-    int Nw()
-    {
-        points();
-        const int n = (int)port_point.size();
-        if (n < 2) return n;
-        const Port_Points &a = port_point[0], &b = port_point[1];
-        const int ax = a.i != b.i ? 0 : a.j != b.j ? 1 : 2;
-        int w = 1;
-        while (w < n && coord(port_point[w], ax) != coord(a, ax) && other_equal(port_point[w], a, ax)) w++;
-        return w;
-    }
+    // points along the port's first in-plane axis (Ports.cpp:235, 256, 277 Nw)
+    int Nw() const { return nw; }
 
 private:
     static double intensity(const Port_Points &q) { return q.Ex * q.Ex + q.Ey * q.Ey + q.Ez * q.Ez; }
-    static int coord(const Port_Points &q, int ax) { return ax == 0 ? q.i : ax == 1 ? q.j : q.k; }
-    static bool other_equal(const Port_Points &q, const Port_Points &a, int ax)
-    {
-        return (ax == 0 || q.i == a.i) && (ax == 1 || q.j == a.j) && (ax == 2 || q.k == a.k);
-    }
     template <class Get>
     void file_rows(const char *f, Get g)  // Ports.cpp:377-388: Nw values per line
     {
         std::ofstream out(f);
-        row(out, g, Nw(), true);
+        points();
+        row(out, g, nw, true);
+    }
+    template <class Get>
+    void file_line(const char *f, Get g)  // Ports.cpp:491-500: every value on one line
+    {
+        std::ofstream out(f);
+        points();
+        row(out, g, (int)port_point.size(), false);
+        out << std::endl;
     }
     template <class Get>
     void step_row(std::ofstream &o, Get g)  // Ports.cpp:389-400: the whole port on one line
@@ -614,6 +626,7 @@ private:
         row(o, g, (int)port_point.size(), false);
         o << std::endl;
     }
+    int nw = 0;
 };
 
 }  // namespace cpsfdtd

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import os
+import shutil
 import subprocess
 import tempfile
 
@@ -311,6 +312,25 @@ def ref_run(case, nsteps, src=None, k0=None):
     for q in range(len(case.ports)):
         ports.append(np.fromfile(os.path.join(out, "port%d_samples.f64" % q), dtype=np.float64).reshape(nsteps, -1, 6))
     return fields, ports
+
+
+def ref_port_text(case, nsteps, src=None):
+    """Run the reference loop with --export; returns {file name: text} of the
+    files its own Ports.cpp Export_* methods wrote."""
+    out = tempfile.mkdtemp(prefix="refdrv_")
+    args = [REFDRV, "run", case.dir, out, str(nsteps), "--export"] + _port_arg(case)
+    if src is not None:
+        args += ["--src"] + [str(x) for x in src]
+    if not case.tfsf:
+        args += ["--notfsf"]
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    texts = {}
+    for name in sorted(os.listdir(out)):
+        if name.endswith(".txt"):
+            with open(os.path.join(out, name)) as f:
+                texts[name] = f.read()
+    shutil.rmtree(out, ignore_errors=True)
+    return texts
 
 
 def ref_bench(case, nsteps, threads=None):

@@ -19,9 +19,17 @@
 //       --k0 v            value the TF_SF object's unset k0 holds
 //       --ports f1,f2     plane port files (p:name) or sphere (s:name)
 //       --notfsf          skip TF/SF even when TF_SF.txt exists
+//       --export          (run) also write each port's text through the
+//                         reference's own Export_* methods: port{q}_Ex.txt and
+//                         port{q}_I.txt (Export_Ex/EIntensity_onPort(ofstream&)
+//                         after every step), and after the last step
+//                         port{q}_E_last.txt, port{q}_Ex_file.txt,
+//                         port{q}_I_file.txt (the const char* forms) and
+//                         port{q}_points.txt (Export_Port)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <new>
 #include <string>
 #include <vector>
@@ -67,6 +75,7 @@ struct Opts {
     double samp = 0;
     int k0set = 0, k0 = 0;
     int notfsf = 0;
+    int exports = 0;
     std::vector<std::string> ports;
 };
 
@@ -82,6 +91,8 @@ static Opts parse_opts(int argc, char **argv, int first)
             o.k0set = 1; o.k0 = atoi(argv[++a]);
         } else if (s == "--notfsf") {
             o.notfsf = 1;
+        } else if (s == "--export") {
+            o.exports = 1;
         } else if (s == "--ports" && a + 1 < argc) {
             std::string l = argv[++a];
             size_t p0 = 0;
@@ -237,16 +248,50 @@ static int cmd_run(const std::string &out, int nsteps, const Opts &o)
 {
     Sim m = setup(o);
     std::vector<std::vector<double>> ps(m.kinds.size());
+    std::vector<std::ofstream *> exs, ins;
+    auto name = [&](size_t q, const char *what) { return out + "/port" + std::to_string(q) + what; };
+    if (o.exports)
+        for (size_t q = 0; q < m.kinds.size(); q++) {
+            exs.push_back(new std::ofstream(name(q, "_Ex.txt")));
+            ins.push_back(new std::ofstream(name(q, "_I.txt")));
+        }
     for (int n = 0; n < nsteps; n++) {
         step(m, o, n);
         size_t ip = 0, is = 0;
         for (size_t q = 0; q < m.kinds.size(); q++) {
             Port_Points *pt;
             int np;
-            if (m.kinds[q] == 0) { PlainPort *P = m.pp[ip++]; P->Update_Fields_on_Port(*m.F); pt = P->port_point; np = P->Num_of_Port_Points - 1; }
-            else { SphericalPort *P = m.sp[is++]; P->Update_Fields_on_Port(*m.F); pt = P->port_point; np = P->Num_of_Port_Points - 1; }
+            if (m.kinds[q] == 0) {
+                PlainPort *P = m.pp[ip++];
+                P->Update_Fields_on_Port(*m.F); pt = P->port_point; np = P->Num_of_Port_Points - 1;
+                if (o.exports) { P->Export_Ex_onPort(*exs[q]); P->Export_EIntensity_onPort(*ins[q]); }
+            } else {
+                SphericalPort *P = m.sp[is++];
+                P->Update_Fields_on_Port(*m.F); pt = P->port_point; np = P->Num_of_Port_Points - 1;
+                if (o.exports) { P->Export_Ex_onPort(*exs[q]); P->Export_EIntensity_onPort(*ins[q]); }
+            }
             for (int t = 0; t < np; t++)
                 for (double v : {pt[t].Ex, pt[t].Ey, pt[t].Ez, pt[t].Hx, pt[t].Hy, pt[t].Hz}) ps[q].push_back(v);
+        }
+    }
+    if (o.exports) {
+        size_t ip = 0, is = 0;
+        for (size_t q = 0; q < m.kinds.size(); q++) {
+            exs[q]->close();
+            ins[q]->close();
+            if (m.kinds[q] == 0) {
+                PlainPort *P = m.pp[ip++];
+                P->Export_E_onPort(name(q, "_E_last.txt").c_str());
+                P->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
+                P->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());
+                P->Export_Port(name(q, "_points.txt").c_str());
+            } else {
+                SphericalPort *P = m.sp[is++];
+                P->Export_E_onPort(name(q, "_E_last.txt").c_str());
+                P->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
+                P->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());
+                P->Export_Port(name(q, "_points.txt").c_str());
+            }
         }
     }
     double ***A[6] = {m.F->Ex, m.F->Ey, m.F->Ez, m.F->Hx, m.F->Hy, m.F->Hz};

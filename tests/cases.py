@@ -57,9 +57,11 @@ def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZER
     return w.run(nt, freqs=freqs, dft_stride=stride, record_ports=record, periodic=periodic, flags=flags)
 
 
-def tiny(directory, nt=40, flags=REF_ZEROHEAP | SRC_REF_PULSE):
+def tiny(directory, nt=40, flags=REF_ZEROHEAP | SRC_REF_PULSE, text_ports=False):
     """24x22x20 grid with every module the reference has; small enough that
-    full field snapshots fit in a fixture."""
+    full field snapshots fit in a fixture.  text_ports adds a 2-D x-normal
+    plane port (6 x 4 points) and a partial sphere port, for the text
+    exports."""
     d = (10e-9, 10e-9, 10e-9)
     n = (24, 22, 20)
     w = CaseWriter(directory)
@@ -75,6 +77,9 @@ def tiny(directory, nt=40, flags=REF_ZEROHEAP | SRC_REF_PULSE):
     w.source(300e-9, 3e-15, 5e-15)
     w.point_source(8, 9, 10, comp=2, amp=2.0)
     w.plane_port_cells("line.txt", "z", 10, 1, 22, 11, 11, d)
+    if text_ports:
+        w.plane_port_cells("face.txt", "x", 16, 4, 9, 5, 8, d)
+        w.sphere_port("cap.txt", 40e-9, 120e-9, 110e-9, 100e-9, 60.0, 120.0, 0.0, 180.0)
     return w.run(nt, record_ports=True, flags=flags)
 
 

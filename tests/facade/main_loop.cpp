@@ -10,10 +10,13 @@
 // --mesh-only stops after Structure::generate_mesh (no device): writes
 // DIR/id.i16 (ID map, [i][j][k]) and prints Nx Ny Nz dx dy dz dt.
 // Writes DIR/field_{Ex..Hz}.f64 (final fields, [i][j][k]), per port q:
-// DIR/port{q}_points.txt (Export_Port), DIR/port{q}_Ex.txt (one line per
-// step, Export_Ex_onPort(ofstream&)), DIR/port{q}_E_last.txt
-// (Export_E_onPort(file) after the last step), DIR/port{q}_samples.f64
-// ([step][point][6] from port_point after Update_Fields_on_Port).
+// DIR/port{q}_points.txt (Export_Port), DIR/port{q}_Ex.txt and
+// DIR/port{q}_I.txt (one line per step, Export_Ex_onPort(ofstream&) and
+// Export_EIntensity_onPort(ofstream&)), after the last step
+// DIR/port{q}_E_last.txt, DIR/port{q}_Ex_file.txt, DIR/port{q}_I_file.txt
+// (Export_E_onPort, Export_Ex_onPort, Export_EIntensity_onPort to a file),
+// and DIR/port{q}_samples.f64 ([step][point][6] from port_point after
+// Update_Fields_on_Port).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -92,10 +95,13 @@ int main(int argc, char **argv)
         for (const std::string &f : spheres) sp.emplace_back(new cpsfdtd::SphericalPort(S, f.c_str()));
 
         const size_t nports = pp.size() + sp.size();
-        std::vector<std::unique_ptr<std::ofstream>> exOut;
+        std::vector<std::unique_ptr<std::ofstream>> exOut, inOut;
         std::vector<std::vector<double>> samples(nports);
-        for (size_t q = 0; q < nports; q++)
-            exOut.emplace_back(new std::ofstream(out + "/port" + std::to_string(q) + "_Ex.txt"));
+        auto name = [&](size_t q, const char *what) { return out + "/port" + std::to_string(q) + what; };
+        for (size_t q = 0; q < nports; q++) {
+            exOut.emplace_back(new std::ofstream(name(q, "_Ex.txt")));
+            inOut.emplace_back(new std::ofstream(name(q, "_I.txt")));
+        }
         for (size_t q = 0; q < pp.size(); q++) pp[q]->Export_Port((out + "/port" + std::to_string(q) + "_points.txt").c_str());
         for (size_t q = 0; q < sp.size(); q++)
             sp[q]->Export_Port((out + "/port" + std::to_string(pp.size() + q) + "_points.txt").c_str());
@@ -115,17 +121,30 @@ int main(int argc, char **argv)
                 cpsfdtd::detail::PortBase &P = q < pp.size() ? (cpsfdtd::detail::PortBase &)*pp[q]
                                                               : (cpsfdtd::detail::PortBase &)*sp[q - pp.size()];
                 P.Update_Fields_on_Port(F);
-                if (q < pp.size()) pp[q]->Export_Ex_onPort(*exOut[q]);
-                else sp[q - pp.size()]->Export_Ex_onPort(*exOut[q]);
+                if (q < pp.size()) {
+                    pp[q]->Export_Ex_onPort(*exOut[q]);
+                    pp[q]->Export_EIntensity_onPort(*inOut[q]);
+                } else {
+                    sp[q - pp.size()]->Export_Ex_onPort(*exOut[q]);
+                    sp[q - pp.size()]->Export_EIntensity_onPort(*inOut[q]);
+                }
                 for (const cpsfdtd::Port_Points &t : P.port_point)
                     samples[q].insert(samples[q].end(), {t.Ex, t.Ey, t.Ez, t.Hx, t.Hy, t.Hz});
             }
         }
         for (size_t q = 0; q < nports; q++) {
             exOut[q]->close();
+            inOut[q]->close();
             cpsfdtd::detail::PortBase &P = q < pp.size() ? (cpsfdtd::detail::PortBase &)*pp[q]
                                                           : (cpsfdtd::detail::PortBase &)*sp[q - pp.size()];
-            P.Export_E_onPort((out + "/port" + std::to_string(q) + "_E_last.txt").c_str());
+            P.Export_E_onPort(name(q, "_E_last.txt").c_str());
+            if (q < pp.size()) {
+                pp[q]->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
+                pp[q]->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());
+            } else {
+                sp[q - pp.size()]->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
+                sp[q - pp.size()]->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());
+            }
             FILE *fh = std::fopen((out + "/port" + std::to_string(q) + "_samples.f64").c_str(), "wb");
             std::fwrite(samples[q].data(), sizeof(double), samples[q].size(), fh);
             std::fclose(fh);

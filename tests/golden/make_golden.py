@@ -13,6 +13,14 @@ Fixtures (all data, no reference source):
                         oblique TF/SF, plane + sphere port), with k0 = 0 and with
                         a nonzero beam reference height (k0 = zs/dz)
   ref_tiny.npz          40 steps of tests.cases.tiny: full fields and port samples
+  ref_port_text.npz     the text the reference's Ports.cpp Export_* methods write
+                        over 40 steps of tests.cases.tiny(text_ports=True) (a
+                        line, a 2-D plane and a partial sphere port), as UTF-8
+                        bytes per file; the entries its loops print from past
+                        the last port point (defect F11: Num_of_Port_Points
+                        counts one point more than it fills, and the plane
+                        exports then read a further row) are cut off here, after
+                        checking that their number is exactly that
 """
 import hashlib
 import os
@@ -41,6 +49,60 @@ def id_digest(ids):
 def tables(prefix, r, out):
     for k, v in r.items():
         out[prefix + k] = v
+
+
+def _values(line):
+    """Tab-terminated values of one exported line."""
+    assert line.endswith("\t"), line[-20:]
+    return line[:-1].split("\t")
+
+
+def _tabbed(values):
+    return "".join(v + "\t" for v in values)
+
+
+def strip_f11(texts, q, sphere, nt):
+    """One port's exports without the entries read past its P filled points:
+    one extra point record (the last line of the point and E lists, one value
+    per row of a sphere port) and, for plane ports, one further row of Nw
+    values in the value exports.  Asserts those counts."""
+    t = {k[len("port%d_" % q):-4]: v for k, v in texts.items() if k.startswith("port%d_" % q)}
+    pts = t["points"].split("\n")
+    assert pts[-1] == "" and pts[-2] == "0\t0\t0"  # the unfilled record, zero-filled heap
+    head = pts[:1] if sphere else []
+    body = pts[1:-2] if sphere else pts[:-2]
+    P = len(body)
+    r = {"points": "\n".join(head + body) + "\n"}
+    last = t["E_last"].split("\n")
+    assert len(last) == P + 2 and last[-1] == ""
+    r["E_last"] = "\n".join(last[:P]) + "\n"
+    if sphere:
+        extra = 1
+        for k in ("Ex_file", "I_file"):  # one line, no newline
+            v = _values(t[k])
+            assert len(v) == P + extra
+            r[k] = _tabbed(v[:P])
+    else:
+        rows = t["Ex_file"].split("\n")
+        nw = len(_values(rows[0]))
+        assert P % nw == 0 and len(rows) == P // nw + 2 and rows[-1] == ""
+        assert all(len(_values(x)) == nw for x in rows[:-1])
+        extra = nw
+        r["Ex_file"] = "".join(x + "\n" for x in rows[:P // nw])
+        assert t["I_file"].endswith("\n")
+        v = _values(t["I_file"][:-1])
+        assert len(v) == P + extra
+        r["I_file"] = _tabbed(v[:P]) + "\n"
+    for k in ("Ex", "I"):  # one line per step
+        lines = t[k].split("\n")
+        assert len(lines) == nt + 1 and lines[-1] == ""
+        cut = []
+        for x in lines[:-1]:
+            v = _values(x)
+            assert len(v) == P + extra
+            cut.append(_tabbed(v[:P]) + "\n")
+        r[k] = "".join(cut)
+    return {"port%d_%s" % (q, k): v for k, v in r.items()}
 
 
 def main():
@@ -92,6 +154,15 @@ def main():
         out[nm] = F[c]
     out["port0"] = P[0]
     np.savez_compressed(os.path.join(HERE, "ref_tiny.npz"), **out)
+
+    d = tempfile.mkdtemp()
+    case = cases.tiny(d, text_ports=True)
+    texts = po.ref_port_text(case, case.nt, src=(8, 9, 10, 2.0))
+    out = {}
+    for q, (_, kind) in enumerate(case.ports):
+        for name, text in strip_f11(texts, q, kind == 1, case.nt).items():
+            out[name] = np.frombuffer(text.encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "ref_port_text.npz"), **out)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import cases
+from port_text import g6
 from cpsfdtd_amd import build as engine_build
 from cpsfdtd_amd.case import FIXED
 
@@ -75,11 +76,6 @@ def test_structure_mesh_matches_oracle(tmp_path, driver, oracle_lib):
     assert np.array_equal(got, want)
 
 
-def g6(v):
-    """std::ostream << double with the default format (%g, precision 6)."""
-    return "%g" % v
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("variant", ["ref", "dl_gain"])
 def test_main_loop_matches_oracle(tmp_path, driver, oracle_lib, variant):
@@ -113,3 +109,23 @@ def test_main_loop_matches_oracle(tmp_path, driver, oracle_lib, variant):
         if case.ports[q][1] == 1:
             lines = lines[1:]  # sphere ports begin with the centre in cells
         assert lines == ["%d\t%d\t%d" % tuple(p) for p in pts]
+
+
+@pytest.mark.gpu
+def test_main_loop_port_text_matches_reference_exports(tmp_path, driver):
+    """Every port text file the facade's Export_* methods write (a line, a 2-D
+    plane and a sphere port over 40 steps) equals, byte for byte, what the
+    reference's own Ports.cpp wrote for the same run
+    (tests/golden/ref_port_text.npz, without the values the reference prints
+    from past the last point, defect F11)."""
+    import port_text
+    g = dict(np.load(os.path.join(ROOT, "tests", "golden", "ref_port_text.npz"), allow_pickle=False))
+    case = cases.tiny(str(tmp_path / "case"), text_ports=True)
+    out = tmp_path / "out"
+    out.mkdir()
+    print("facade port text", flush=True)
+    subprocess.run(driver_args(case, case.nt, out), check=True, timeout=120)
+    for q in range(len(case.ports)):
+        want = port_text.fixture(g, q)
+        for k in port_text.KEYS:
+            assert (out / ("port%d_%s.txt" % (q, k))).read_text() == want[k], (q, k)
