@@ -110,6 +110,10 @@ SIGNATURES = {
     "cfd_server_last_error": ([], C.c_char_p),
     "cfd_port_flux": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "cfd_dft_flux": ([_dp, C.c_int, C.c_int, C.c_int, C.c_double, _dp], C.c_int),
+    "cfd_image_colormap": ([_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_ubyte)], C.c_int),
+    "cfd_image_materials": ([C.POINTER(C.c_short), C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
+    "cfd_image_write_png": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
+    "cfd_image_last_error": ([], C.c_char_p),
 }
 
 _lib = None
@@ -403,6 +407,46 @@ class Batch:
 
     def kernel_times(self):
         return _kernel_times(library().cfd_batch_kernel_times, self.h)
+
+
+def _image_check(rc):
+    if rc < 0:
+        raise EngineError(library().cfd_image_last_error().decode())
+
+
+def image_colormap(values, cmin, cmax, rgb=None):
+    """cfd_image_colormap (OutImage::plot_save's colour map) of a map
+    values[x][y]; returns pixels [y][x][3] (uint8).  Updates rgb in place when
+    given (NaN values leave their pixels as they were)."""
+    v = np.ascontiguousarray(values, np.float64)
+    nx, ny = v.shape
+    if rgb is None:
+        rgb = np.zeros((ny, nx, 3), np.uint8)
+    if rgb.shape != (ny, nx, 3) or rgb.dtype != np.uint8 or not rgb.flags.c_contiguous:
+        raise ValueError("rgb must be a contiguous uint8 array [ny][nx][3]")
+    _image_check(library().cfd_image_colormap(v.ctypes.data_as(_dp), nx, ny, float(cmin), float(cmax),
+                                              rgb.ctypes.data_as(C.POINTER(C.c_ubyte))))
+    return rgb
+
+
+def image_materials(ids):
+    """cfd_image_materials (OutImage::structure_plot_save) of ids[x][y];
+    returns pixels [y][x][3] (uint8)."""
+    v = np.ascontiguousarray(ids, np.int16)
+    nx, ny = v.shape
+    rgb = np.zeros((ny, nx, 3), np.uint8)
+    _image_check(library().cfd_image_materials(v.ctypes.data_as(C.POINTER(C.c_short)), nx, ny,
+                                               rgb.ctypes.data_as(C.POINTER(C.c_ubyte))))
+    return rgb
+
+
+def write_png(path, rgb):
+    """cfd_image_write_png: pixels [height][width][3] (uint8) to an RGB PNG."""
+    p = np.ascontiguousarray(rgb, np.uint8)
+    if p.ndim != 3 or p.shape[2] != 3:
+        raise ValueError("rgb must be [height][width][3]")
+    _image_check(library().cfd_image_write_png(os.fsencode(path), p.shape[1], p.shape[0],
+                                               p.ctypes.data_as(C.POINTER(C.c_ubyte))))
 
 
 def dft_flux(dft, axis, scale):

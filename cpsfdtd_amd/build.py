@@ -26,7 +26,7 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-I" + os.path.join
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 
-HOST_SOURCES = ["model.cpp", "model_capi.cpp", "server.cpp"]
+HOST_SOURCES = ["model.cpp", "model_capi.cpp", "server.cpp", "image.cpp"]
 HIP_SOURCES = ["engine.hip"]
 HEADERS = ["model.hpp", "engine.hpp", "kernels.hip"]
 
@@ -56,7 +56,7 @@ def build(verbose=False, force=True):
         _run([HIPCC] + COMMON + HIP_FLAGS + ["-c", os.path.join(CSRC, f), "-o", o], verbose)
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs, verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + ["-lz"], verbose)
     os.replace(tmp, LIB)
     build_facade_driver(verbose)
     return LIB

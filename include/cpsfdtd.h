@@ -205,6 +205,20 @@ int cfd_model_port(cfd_model *m, int port, int *ijk);               /* returns p
 /* DL / gain cell lists in reference [i][j][k] flat order; returns count */
 int cfd_model_cells(cfd_model *m, int gain, long long *flat);
 
+/* Image export of a plane port's maps, host only (OutImage.cpp; the
+ * reference draws with CImg and saves with CImg::save_png).  Maps are
+ * [nx][ny] with x along the image width (PlainPort::Export_EIntensity /
+ * Export_ID layout, Ports.cpp:322-340); pixels are rgb[ny][nx][3]. */
+/* OutImage::plot_save colour map (OutImage.cpp:140-202): updates rgb in place
+ * (a NaN value leaves its pixel unchanged, as in the reference) */
+int cfd_image_colormap(const double *map, int nx, int ny, double cmin, double cmax, unsigned char *rgb);
+/* OutImage::structure_plot_save colour table (OutImage.cpp:44-53, 204-217);
+ * an ID outside 0-9 is an error */
+int cfd_image_materials(const short *ids, int nx, int ny, unsigned char *rgb);
+/* CImg::save_png of those pixels (OutImage.cpp:199, 215): 8-bit RGB PNG */
+int cfd_image_write_png(const char *path, int width, int height, const unsigned char *rgb);
+const char *cfd_image_last_error(void);
+
 /* Per-kernel timing of the last cfd_step call (HIP events on the sim's stream):
  * names[i] (<=32 chars) and total milliseconds; returns entries. */
 int cfd_kernel_times(cfd_sim *s, char (*names)[32], double *ms, int *launches, int cap);

@@ -44,6 +44,7 @@
 #define CPSFDTD_HPP
 
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <memory>
 #include <sstream>
@@ -566,25 +567,37 @@ private:
 
 class PlainPort : public detail::PortBase {
 public:
-    PlainPort(Structure &sS, const char *fileName) : PortBase(sS, fileName, CFD_PORT_PLANE)
+    // The Structure must outlive the port (Export_ID reads its ID map).
+    PlainPort(Structure &sS, const char *fileName) : PortBase(sS, fileName, CFD_PORT_PLANE), S(&sS)
     {
-        // Ports.cpp:211-283: normal, position, w1 w2, h1 h2; the first
-        // in-plane index runs over (int)(w1/dw)..(int)(w2/dw), dw = dy for an
-        // x-normal port, else dx
+        // Ports.cpp:211-283: normal, position, w1 w2, h1 h2; the in-plane
+        // indices run over (int)(w1/dw)..(int)(w2/dw) and (int)(h1/dh)..
+        // (int)(h2/dh), with (dw, dh) = (dy, dz) for an x-normal port,
+        // (dx, dz) for y, (dx, dy) for z
         std::ifstream in(fileName);
         std::string line;
-        char normal = 0;
-        double position = 0, w1 = 0, w2 = 0;
         std::getline(in, line);
-        std::istringstream(line) >> normal;
+        std::istringstream(line) >> normal_direction;
         std::getline(in, line);
         std::istringstream(line) >> position;
         std::getline(in, line);
         std::istringstream(line) >> w1 >> w2;
-        const double dw = normal == 'x' ? sS.dy : sS.dx;
-        if (!(dw > 0)) throw Error("PlainPort: construct it after Structure::generate_mesh");
-        nw = (int)(w2 / dw) - (int)(w1 / dw) + 1;
+        std::getline(in, line);
+        std::istringstream(line) >> h1 >> h2;
+        const double dw = normal_direction == 'x' ? sS.dy : sS.dx;
+        const double dh = normal_direction == 'z' ? sS.dy : sS.dz;
+        if (!(dw > 0) || !(dh > 0)) throw Error("PlainPort: construct it after Structure::generate_mesh");
+        Nw = (int)(w2 / dw) - (int)(w1 / dw) + 1;
+        Nh = (int)(h2 / dh) - (int)(h1 / dh) + 1;
+        if (Nw < 1 || Nh < 1) throw Error(std::string("PlainPort: empty port in ") + fileName);
+        out_store.assign((size_t)Nw * Nh, 0.0);  // Ports.cpp:296-303: port_out [Nw][Nh], zeroed
+        out_rows.resize(Nw);
+        for (int i = 0; i < Nw; ++i) out_rows[i] = out_store.data() + (size_t)i * Nh;
+        port_out = out_rows.data();
     }
+    PlainPort(const PlainPort &) = delete;
+    PlainPort &operator=(const PlainPort &) = delete;
+
     void Export_Port(const char *fileName)  // Ports.cpp:355-362
     {
         std::ofstream out(fileName);
@@ -600,17 +613,67 @@ public:
     void Export_EIntensity_onPort(std::ofstream &o) { step_row(o, intensity); }
     using PortBase::Export_E_onPort;
 
-    // points along the port's first in-plane axis (Ports.cpp:235, 256, 277 Nw)
-    int Nw() const { return nw; }
+    // Ports.cpp:308-320: the material IDs, Nw per line
+    void Export_ID_onPort(const char *fileName)
+    {
+        std::ofstream out(fileName);
+        short **m = Export_ID();
+        for (int j = 0; j < Nh; ++j) {
+            for (int i = 0; i < Nw; ++i) out << m[i][j] << "\t";
+            out << std::endl;
+        }
+    }
+    // Ports.cpp:322-340: [Nw][Nh] material IDs; the array belongs to the port
+    // and holds until the next Export_ID call
+    short **Export_ID()
+    {
+        points();
+        check_shape();
+        id_store.resize(port_point.size());
+        id_rows.resize(Nw);
+        for (int i = 0; i < Nw; ++i) id_rows[i] = id_store.data() + (size_t)i * Nh;
+        for (size_t n = 0; n < port_point.size(); ++n) {
+            const Port_Points &q = port_point[n];
+            id_rows[n % Nw][n / Nw] = S->ID[q.i][q.j][q.k];
+        }
+        return id_rows.data();
+    }
+    // Ports.cpp:405-415, 442-452, 479-489, 512-522: fill port_out [Nw][Nh]
+    // from the last Update_Fields_on_Port and return it
+    double **Export_Ex() { return map([](const Port_Points &q) { return q.Ex; }); }
+    double **Export_Ey() { return map([](const Port_Points &q) { return q.Ey; }); }
+    double **Export_Ez() { return map([](const Port_Points &q) { return q.Ez; }); }
+    double **Export_EIntensity() { return map(intensity); }
+
+    // the reference's public members (Ports.h:94-98); point n of port_point
+    // sits at (n % Nw, n / Nw) of the in-plane grid
+    char normal_direction = 0;
+    double position = 0, w1 = 0, w2 = 0, h1 = 0, h2 = 0;
+    double **port_out = nullptr;
+    int Nw = 0, Nh = 0;
 
 private:
+    void check_shape() const
+    {
+        if (port_point.size() != (size_t)Nw * Nh)
+            throw Error("PlainPort: " + std::to_string(port_point.size()) + " points, not Nw x Nh = " +
+                        std::to_string(Nw) + " x " + std::to_string(Nh));
+    }
+    template <class Get>
+    double **map(Get g)
+    {
+        points();
+        check_shape();
+        for (size_t n = 0; n < port_point.size(); ++n) out_rows[n % Nw][n / Nw] = g(port_point[n]);
+        return port_out;
+    }
     static double intensity(const Port_Points &q) { return q.Ex * q.Ex + q.Ey * q.Ey + q.Ez * q.Ez; }
     template <class Get>
     void file_rows(const char *f, Get g)  // Ports.cpp:377-388: Nw values per line
     {
         std::ofstream out(f);
         points();
-        row(out, g, nw, true);
+        row(out, g, Nw, true);
     }
     template <class Get>
     void file_line(const char *f, Get g)  // Ports.cpp:491-500: every value on one line
@@ -626,7 +689,68 @@ private:
         row(o, g, (int)port_point.size(), false);
         o << std::endl;
     }
-    int nw = 0;
+    const Structure *S;
+    std::vector<double> out_store;
+    std::vector<double *> out_rows;
+    std::vector<short> id_store;
+    std::vector<short *> id_rows;
+};
+
+// ---------------------------------------------------------------------------
+// OutImage (OutImage.h:21-41): colour maps of [Nx][Ny] maps (a plane port's
+// Export_EIntensity / Export_ID) written as PNG files.  There is no display
+// window: plot() only updates the image; plot_save() also writes
+// "<base>NNNNNN.png" (NNNNNN = time_step / save_modulus, OutImage.cpp:198),
+// structure_plot_save() writes "<base>_structure.png" (OutImage.cpp:214).
+// ---------------------------------------------------------------------------
+class OutImage {
+public:
+    OutImage(int N_x, int N_y, int save_modulus, double cmap_min, double cmap_max, const char *file_name_base)
+        : savemodulus(save_modulus), Nx(N_x), Ny(N_y), cmapmin(cmap_min), cmapmax(cmap_max),
+          filenamebase(file_name_base ? file_name_base : "")
+    {
+        if (Nx <= 0 || Ny <= 0 || savemodulus <= 0) throw Error("OutImage: sizes and save modulus must be positive");
+        rgb.assign((size_t)Nx * Ny * 3, 0);
+    }
+    void plot(double **dD, int time_step)
+    {
+        if (time_step % savemodulus == 0) draw(dD);
+    }
+    void plot_save(double **dD, int time_step)
+    {
+        if (time_step % savemodulus != 0) return;
+        draw(dD);
+        char num[16];
+        std::snprintf(num, sizeof num, "%06d", time_step / savemodulus);
+        save(filenamebase + num + ".png");
+    }
+    void structure_plot_save(short **dD)
+    {
+        std::vector<short> v((size_t)Nx * Ny);
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j) v[(size_t)i * Ny + j] = dD[i][j];
+        image_check(cfd_image_materials(v.data(), Nx, Ny, rgb.data()));
+        save(filenamebase + "_structure.png");
+    }
+
+    int savemodulus, Nx, Ny;
+    double cmapmin, cmapmax;
+    std::string filenamebase;
+    std::vector<unsigned char> rgb;  // [Ny][Nx][3], the image's current pixels
+
+private:
+    static void image_check(int rc)
+    {
+        if (rc < 0) throw Error(cfd_image_last_error());
+    }
+    void draw(double **dD)
+    {
+        std::vector<double> v((size_t)Nx * Ny);
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j) v[(size_t)i * Ny + j] = dD[i][j];
+        image_check(cfd_image_colormap(v.data(), Nx, Ny, cmapmin, cmapmax, rgb.data()));
+    }
+    void save(const std::string &path) { image_check(cfd_image_write_png(path.c_str(), Nx, Ny, rgb.data())); }
 };
 
 }  // namespace cpsfdtd

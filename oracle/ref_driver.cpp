@@ -24,8 +24,9 @@
 //                         port{q}_I.txt (Export_Ex/EIntensity_onPort(ofstream&)
 //                         after every step), and after the last step
 //                         port{q}_E_last.txt, port{q}_Ex_file.txt,
-//                         port{q}_I_file.txt (the const char* forms) and
-//                         port{q}_points.txt (Export_Port)
+//                         port{q}_I_file.txt (the const char* forms),
+//                         port{q}_points.txt (Export_Port) and, for plane
+//                         ports, port{q}_ID_file.txt (Export_ID_onPort)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -285,6 +286,7 @@ static int cmd_run(const std::string &out, int nsteps, const Opts &o)
                 P->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
                 P->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());
                 P->Export_Port(name(q, "_points.txt").c_str());
+                P->Export_ID_onPort(name(q, "_ID_file.txt").c_str());
             } else {
                 SphericalPort *P = m.sp[is++];
                 P->Export_E_onPort(name(q, "_E_last.txt").c_str());

@@ -5,7 +5,7 @@
 //   main_loop --material F --structure F [--pml F] [--tfsf F] [--dl F]
 //             [--gain F] [--source F] [--plane F]... [--sphere F]...
 //             [--src i j k comp amp] [--steps N] [--flags U] [--out DIR]
-//             [--mesh-only]
+//             [--mesh-only] [--images CMAX]
 //
 // --mesh-only stops after Structure::generate_mesh (no device): writes
 // DIR/id.i16 (ID map, [i][j][k]) and prints Nx Ny Nz dx dy dz dt.
@@ -16,7 +16,11 @@
 // DIR/port{q}_E_last.txt, DIR/port{q}_Ex_file.txt, DIR/port{q}_I_file.txt
 // (Export_E_onPort, Export_Ex_onPort, Export_EIntensity_onPort to a file),
 // and DIR/port{q}_samples.f64 ([step][point][6] from port_point after
-// Update_Fields_on_Port).
+// Update_Fields_on_Port); plane ports also DIR/port{q}_ID_file.txt
+// (Export_ID_onPort).  --images CMAX: for each plane port, OutImage over its
+// Nw x Nh map with colour range [0, CMAX] as main.cpp:149-160, 228 uses it:
+// DIR/port{q}_img_structure.png (structure_plot_save(Export_ID())) and
+// DIR/port{q}_img{steps:06d}.png (plot_save(Export_EIntensity(), steps)).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -36,6 +40,7 @@ int main(int argc, char **argv)
     int steps = 10;
     unsigned flags = 0;
     bool mesh_only = false;
+    double images_cmax = -1;
     for (int a = 1; a < argc; a++) {
         std::string k = argv[a];
         auto next = [&]() -> std::string {
@@ -55,6 +60,7 @@ int main(int argc, char **argv)
         else if (k == "--flags") flags = (unsigned)std::strtoul(next().c_str(), nullptr, 0);
         else if (k == "--out") out = next();
         else if (k == "--mesh-only") mesh_only = true;
+        else if (k == "--images") images_cmax = std::atof(next().c_str());
         else if (k == "--src") {
             for (int q = 0; q < 4; q++) src_ijkc.push_back(std::atoi(next().c_str()));
             src_amp.push_back(std::atof(next().c_str()));
@@ -141,6 +147,12 @@ int main(int argc, char **argv)
             if (q < pp.size()) {
                 pp[q]->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
                 pp[q]->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());
+                pp[q]->Export_ID_onPort(name(q, "_ID_file.txt").c_str());
+                if (images_cmax > 0) {
+                    cpsfdtd::OutImage img(pp[q]->Nw, pp[q]->Nh, 1, 0, images_cmax, name(q, "_img").c_str());
+                    img.structure_plot_save(pp[q]->Export_ID());
+                    img.plot_save(pp[q]->Export_EIntensity(), steps);
+                }
             } else {
                 sp[q - pp.size()]->Export_Ex_onPort(name(q, "_Ex_file.txt").c_str());
                 sp[q - pp.size()]->Export_EIntensity_onPort(name(q, "_I_file.txt").c_str());

@@ -15,7 +15,8 @@ Fixtures (all data, no reference source):
   ref_tiny.npz          40 steps of tests.cases.tiny: full fields and port samples
   ref_port_text.npz     the text the reference's Ports.cpp Export_* methods write
                         over 40 steps of tests.cases.tiny(text_ports=True) (a
-                        line, a 2-D plane and a partial sphere port), as UTF-8
+                        line, a 2-D plane and a partial sphere port; plane
+                        ports also Export_ID_onPort), as UTF-8
                         bytes per file; the entries its loops print from past
                         the last port point (defect F11: Num_of_Port_Points
                         counts one point more than it fills, and the plane
@@ -89,6 +90,10 @@ def strip_f11(texts, q, sphere, nt):
         assert all(len(_values(x)) == nw for x in rows[:-1])
         extra = nw
         r["Ex_file"] = "".join(x + "\n" for x in rows[:P // nw])
+        rows = t["ID_file"].split("\n")  # same rows of Nw
+        assert len(rows) == P // nw + 2 and rows[-1] == ""
+        assert all(len(_values(x)) == nw for x in rows[:-1])
+        r["ID_file"] = "".join(x + "\n" for x in rows[:P // nw])
         assert t["I_file"].endswith("\n")
         v = _values(t["I_file"][:-1])
         assert len(v) == P + extra

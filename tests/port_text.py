@@ -12,12 +12,15 @@ Layouts (P points, samples [steps][P][6]):
            (Ports.cpp:120-127), plane ports Nw values per line (Ports.cpp:366-377)
   I_file   Export_EIntensity_onPort(file): sphere one line without a newline
            (Ports.cpp:172-179), plane one line with a newline (Ports.cpp:491-500)
+  ID_file  plane ports, Export_ID_onPort: the material IDs at the points,
+           Nw per line (Ports.cpp:308-320)
 The committed fixture tests/golden/ref_port_text.npz holds what the
 reference's own Ports.cpp writes; these functions are checked against it.
 """
 import numpy as np
 
 KEYS = ("points", "E_last", "Ex", "I", "Ex_file", "I_file")
+PLANE_KEYS = KEYS + ("ID_file",)
 
 
 def g6(v):
@@ -44,8 +47,9 @@ def sphere_centre(path, d):
     return xc / d[0], yc / d[1], zc / d[2]
 
 
-def exports(samples, points, sphere, nw=None, centre=None):
-    """{key: text} for one port after len(samples) steps."""
+def exports(samples, points, sphere, nw=None, centre=None, ids=None):
+    """{key: text} for one port after len(samples) steps; ids: the material
+    ID at each point (plane ports)."""
     s = np.asarray(samples, np.float64)
     ex = s[:, :, 0]
     inten = s[:, :, 0] * s[:, :, 0] + s[:, :, 1] * s[:, :, 1] + s[:, :, 2] * s[:, :, 2]
@@ -61,9 +65,11 @@ def exports(samples, points, sphere, nw=None, centre=None):
         n = s.shape[1]
         r["Ex_file"] = "".join(_tabbed(ex[-1][i:i + nw]) + "\n" for i in range(0, n, nw))
         r["I_file"] = _tabbed(inten[-1]) + "\n"
+        if ids is not None:
+            r["ID_file"] = "".join("".join("%d\t" % v for v in ids[i:i + nw]) + "\n" for i in range(0, n, nw))
     return r
 
 
 def fixture(g, q):
     """{key: text} of port q from the loaded ref_port_text.npz."""
-    return {k: bytes(g["port%d_%s" % (q, k)]).decode() for k in KEYS}
+    return {k: bytes(g["port%d_%s" % (q, k)]).decode() for k in PLANE_KEYS if "port%d_%s" % (q, k) in g}

@@ -125,10 +125,14 @@ def test_port_text_matches_reference_exports(oracle_lib, tmp_path):
     assert len(case.ports) == 3
     for q, (path, kind) in enumerate(case.ports):
         sphere = kind == 1
-        got = port_text.exports(o.port_samples(q), o.port_points(q), sphere,
+        pts = o.port_points(q)
+        ids = o.id_map()[pts[:, 0], pts[:, 1], pts[:, 2]]
+        got = port_text.exports(o.port_samples(q), pts, sphere,
                                 nw=None if sphere else port_text.plane_nw(path, o.d),
-                                centre=port_text.sphere_centre(path, o.d) if sphere else None)
+                                centre=port_text.sphere_centre(path, o.d) if sphere else None, ids=ids)
         want = port_text.fixture(g, q)
-        for k in port_text.KEYS:
+        assert sorted(want) == sorted(port_text.KEYS if sphere else port_text.PLANE_KEYS)
+        for k in want:
             assert got[k] == want[k], (q, k)
+    assert len(set(bytes(g["port0_ID_file"]).decode().split())) > 1  # the line crosses two media
     assert "\n" in port_text.fixture(g, 1)["Ex_file"].rstrip("\n")  # the 2-D plane prints rows of Nw
