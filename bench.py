@@ -174,10 +174,42 @@ def spectrum_peak(freqs, spectrum):
     return float(f[k]), q
 
 
+def lorentz_fit(freqs, spectrum):
+    """Peak frequency and Q = f0 / FWHM of a Lorentzian A / (1 + ((f - f0)/g)^2)
+    fitted by least squares to the spectrum's samples above half maximum
+    around its peak (FWHM = 2g); None, None when the half-maximum window is
+    not closed on both sides, has fewer than four samples, or the fit does
+    not converge."""
+    from scipy.optimize import curve_fit
+    f = np.asarray(freqs, np.float64)
+    p = np.asarray(spectrum, np.float64)
+    order = np.argsort(f)
+    f, p = f[order], p[order]
+    k = int(np.argmax(p))
+    lo = k
+    while lo > 0 and p[lo - 1] >= p[k] / 2:
+        lo -= 1
+    hi = k
+    while hi < len(p) - 1 and p[hi + 1] >= p[k] / 2:
+        hi += 1
+    if lo == 0 or hi == len(p) - 1 or hi - lo + 1 < 4:  # not bracketed, or too few samples
+        return None, None
+    x, y = f[lo:hi + 1] / f[k], p[lo:hi + 1] / p[k]  # unit scale for the solver
+    g0 = max((x[-1] - x[0]) / 2, 1e-3)
+    try:
+        (a, f0, g), _ = curve_fit(lambda v, a, f0, g: a / (1 + ((v - f0) / g) ** 2), x, y, p0=(1.0, 1.0, g0),
+                                  maxfev=2000)
+    except (RuntimeError, ValueError):
+        return None, None
+    if not (np.isfinite(f0) and np.isfinite(g) and g != 0):
+        return None, None
+    return float(f0 * f[k]), float(f0 / (2 * abs(g)))
+
+
 def quality(res32, res64, freqs):
     """FP32 against FP64 over the same run: relative L2 of every port's |E|^2
     time series, DFT and flux spectrum, and the transmitted port's flux
-    spectrum peak and Q."""
+    spectrum peak and Q (read at half maximum, and from a Lorentzian fit)."""
 
     def rel(a, b):
         den = np.linalg.norm(np.ravel(b))
@@ -190,6 +222,9 @@ def quality(res32, res64, freqs):
         f64, q64 = spectrum_peak(freqs, res64[-1][2])
         out.update({"peak_thz_f32": round(f32 / 1e12, 4), "peak_thz_f64": round(f64 / 1e12, 4),
                     "q_f32": q32 and round(q32, 4), "q_f64": q64 and round(q64, 4)})
+        for prec, res in ((32, res32), (64, res64)):
+            f0, q = lorentz_fit(freqs, res[-1][2])
+            out["lorentz_f%d" % prec] = {"peak_thz": f0 and round(f0 / 1e12, 4), "q": q and round(q, 4)}
     return out
 
 
@@ -218,6 +253,25 @@ def cpu_baseline(scale_steps=200):
     sec = time.perf_counter() - t0
     return {"value": round(256 ** 3 * 3 / sec / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
             "sample": "3 steps of the same workload on the FP64 oracle, %.1f s" % sec}
+
+
+def cpu_variants():
+    """The reference loop's rate on the same 256^3 grid at the reference
+    makefile's optimisation (-O0) and at -O2, on one thread and on all of
+    this process's cores; each a bounded sample (10-200 steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from cpsfdtd_amd import configs
+    threads = len(os.sched_getaffinity(0))
+    out = []
+    for opt, nth, steps in (("O2", threads, 200), ("O2", 1, 20), ("O0", threads, 40), ("O0", 1, 10)):
+        d = tempfile.mkdtemp(prefix="bench_ref_")
+        case = configs.c2_reference_core(d, nt=steps + 2)
+        r = pyoracle.ref_bench(case, steps, threads=nth, opt=opt)
+        out.append({"opt": "-" + opt, "threads": r["threads"], "steps": steps, "seconds": round(r["seconds"], 2),
+                    "mcells_per_s": round(r["mcells_per_s"], 3)})
+        log("bench: reference %s, %d threads: %.1f Mcell-upd/s" % (opt, r["threads"], r["mcells_per_s"]))
+    return out
 
 
 def batched_c4(steps=200, warmup=10, n=512):
@@ -305,6 +359,23 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0):
             "policy": "shortest work first" if policy == eng.POLICY_SHORTEST else "first come, first served"}
 
 
+def cpu_fifo_model(nreq, mcells_per_s):
+    """Latencies of the same request stream solved one after another, first
+    come first served, by a CPU at a fixed rate (the measured reference rate
+    on the same kind of grid): service time = Nx Ny Nz steps / rate."""
+    from cpsfdtd_amd import configs
+    rq = configs.c5_requests(nreq)
+    free, lat = 0.0, []
+    for t, p in rq:
+        done = max(t, free) + p["N"][0] * p["N"][1] * p["N"][2] * p["steps"] / (mcells_per_s * 1e6)
+        lat.append(done - t)
+        free = done
+    lat = np.array(lat)
+    return {"p50_s": round(float(np.percentile(lat, 50)), 1), "p99_s": round(float(np.percentile(lat, 99)), 1),
+            "makespan_s": round(free - rq[0][0], 1), "rate_mcells": mcells_per_s,
+            "basis": "sequential FIFO solve at the CPU baseline's measured rate (a model, not a run)"}
+
+
 def other_configs(steps=400, warmup=20):
     """The single-grid workloads beside c2, each in FP32 and FP64: c1 (128^3
     sphere, Ez source, line port recording every step) and c3 (192x192x64
@@ -355,7 +426,13 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--cpu-variants", action="store_true",
+                    help="only time the reference loop at -O0 / -O2 on 1 / all threads (no GPU) and print it")
     args = ap.parse_args()
+    if args.cpu_variants:
+        print(json.dumps({"cpu_variants": cpu_variants(), "workload": "256^3 grid of the c2 workload, CPML + TF/SF, "
+                          "FP64, the reference's own sources"}), flush=True)
+        return
     if args.copy_calib:
         import torch
         n = 1 << 28
@@ -476,6 +553,9 @@ def main():
             out["cpu_baseline"] = cpu_baseline()
         except Exception as ex:  # reported, never fatal for the GPU figure
             out["cpu_baseline"] = {"error": str(ex)}
+        rate = out["cpu_baseline"].get("value")
+        if rate and "p50_s" in out.get("serving", {}):
+            out["serving"]["cpu_fifo_model"] = cpu_fifo_model(args.serve_requests, rate)
     print(json.dumps(out), flush=True)
 
 

@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REFDRV = os.path.join(HERE, "_ref", "refdrv")
+REFDRV_O0 = os.path.join(HERE, "_ref", "refdrv_O0")  # reference sources at -O0 (timing only)
 
 
 def build(quiet=True):
@@ -333,11 +334,13 @@ def ref_port_text(case, nsteps, src=None):
     return texts
 
 
-def ref_bench(case, nsteps, threads=None):
+def ref_bench(case, nsteps, threads=None, opt="O2"):
+    """Time the reference loop (sources at -O2, or -O0 with opt="O0")."""
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
-    args = [REFDRV, "bench", case.dir, str(nsteps)] + (["--notfsf"] if not case.tfsf else [])
+    drv = REFDRV_O0 if opt == "O0" else REFDRV
+    args = [drv, "bench", case.dir, str(nsteps)] + (["--notfsf"] if not case.tfsf else [])
     r = subprocess.run(args, check=True, capture_output=True, text=True, env=env)
     line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
     return json.loads(line)
