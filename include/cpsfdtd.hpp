@@ -43,6 +43,7 @@
 #ifndef CPSFDTD_HPP
 #define CPSFDTD_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -490,13 +491,41 @@ protected:
         ctx->port_files.push_back(fileName);
         ctx->port_kinds.push_back(kind);
     }
+    // The point list depends on the mesh and the port file only.  Before the
+    // device simulation exists (main.cpp:144-151 exports the sphere ports'
+    // lists right after constructing them, then constructs the plane ports), it comes
+    // from the host model of this port alone, so asking for it does not start
+    // the simulation; afterwards from the simulation.  Both are the same
+    // computation.
     void points()
     {
-        if (!port_point.empty() || !ctx->get()) return;
-        const int n = cfd_port_npts(ctx->sim, index);
-        check(n, "cfd_port_npts");
-        std::vector<int> ijk((size_t)n * 3);
-        if (n) check(cfd_port_points(ctx->sim, index, ijk.data()), "cfd_port_points");
+        if (!port_point.empty()) return;
+        std::vector<int> ijk;
+        int n = 0;
+        if (ctx->sim) {
+            n = cfd_port_npts(ctx->sim, index);
+            check(n, "cfd_port_npts");
+            ijk.resize((size_t)n * 3);
+            if (n) check(cfd_port_points(ctx->sim, index, ijk.data()), "cfd_port_points");
+        } else {
+            cfd_config c{};
+            const char *file = ctx->port_files[index].c_str();
+            int kind = ctx->port_kinds[index];
+            c.material = ctx->material.c_str();
+            c.structure = ctx->structure.c_str();
+            c.n_ports = 1;
+            c.port_files = &file;
+            c.port_kinds = &kind;
+            c.nt = 1;
+            c.precision = CFD_F64;
+            cfd_model *m = nullptr;
+            if (cfd_model_build(&c, &m) < 0) throw Error(std::string("port ") + file + ": " + cfd_model_last_error());
+            n = cfd_model_port(m, 0, nullptr);
+            ijk.resize((size_t)std::max(n, 0) * 3);
+            if (n > 0) cfd_model_port(m, 0, ijk.data());
+            cfd_model_free(m);
+            if (n < 0) throw Error(std::string("port ") + file + ": no point list");
+        }
         port_point.resize(n);
         for (int t = 0; t < n; t++) port_point[t] = Port_Points{ijk[3 * t], ijk[3 * t + 1], ijk[3 * t + 2], 0, 0, 0, 0, 0, 0};
         Num_of_Port_Points = n;

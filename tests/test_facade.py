@@ -61,6 +61,55 @@ def test_header_compiles_standalone(tmp_path):
                     "-I" + os.path.join(ROOT, "include"), str(src)], check=True)
 
 
+def test_port_exports_before_the_run_follow_main_cpp_order(tmp_path):
+    """main.cpp:144-160 exports the sphere ports' point lists right after
+    constructing them, then constructs the plane ports and saves their
+    structure images, all before the first update.  Through the facade this
+    runs host-side only (no simulation is started, so later constructors
+    still register), and writes what the reference's Ports.cpp wrote for the
+    same ports (tests/golden/ref_port_text.npz) and the OutImage structure map
+    of those IDs."""
+    import port_text
+    from test_image import decode_png
+    if not os.path.exists(engine_build.LIB):
+        pytest.skip("libcpsfdtd.so is not built")
+    case = cases.tiny(str(tmp_path / "case"), text_ports=True)
+    src = tmp_path / "order.cpp"
+    src.write_text(r'''
+#include "cpsfdtd.hpp"
+using namespace cpsfdtd;
+int main(int, char **argv)
+{
+    const std::string d = argv[1], out = argv[2];
+    Structure S((d + "/material.txt").c_str());
+    S.generate_mesh((d + "/structure.txt").c_str());
+    S.InitializeBasicFDTDCoefficients();
+    SphericalPort cap(S, (d + "/cap.txt").c_str());
+    cap.Export_Port((out + "/cap_points.txt").c_str());
+    PlainPort line(S, (d + "/line.txt").c_str());
+    PlainPort face(S, (d + "/face.txt").c_str());
+    face.Export_Port((out + "/face_points.txt").c_str());
+    line.Export_ID_onPort((out + "/line_ID.txt").c_str());
+    OutImage img(line.Nw, line.Nh, 1, 0, 0.02, (out + "/line").c_str());
+    img.structure_plot_save(line.Export_ID());
+    return S.ctx->sim ? 3 : 0;
+}
+''')
+    exe = tmp_path / "order"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L" + engine_build.LIBDIR, "-lcpsfdtd", "-Wl,-rpath," + engine_build.LIBDIR], check=True)
+    out = tmp_path / "out"
+    out.mkdir()
+    subprocess.run([str(exe), case.dir, str(out)], check=True, timeout=60)
+    g = dict(np.load(os.path.join(ROOT, "tests", "golden", "ref_port_text.npz"), allow_pickle=False))
+    line, face, cap = (port_text.fixture(g, q) for q in range(3))
+    assert (out / "cap_points.txt").read_text() == cap["points"]
+    assert (out / "face_points.txt").read_text() == face["points"]
+    assert (out / "line_ID.txt").read_text() == line["ID_file"]
+    ids = np.array([[int(v) for v in line["ID_file"].split()]], np.int16).T  # [Nw][Nh = 1]
+    assert np.array_equal(decode_png(str(out / "line_structure.png")), eng.image_materials(ids))
+
+
 def test_structure_mesh_matches_oracle(tmp_path, driver, oracle_lib):
     """Structure(material) + generate_mesh(structure) through the facade, no
     device: dimensions, time step and the ID map equal the oracle's."""
