@@ -44,6 +44,7 @@
 #define CPSFDTD_HPP
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -370,7 +371,78 @@ public:
     {
         ctx->require_unstarted("TF_SF");
         ctx->tfsf = fileName;
+        // the reference's public parameters (TF_SF.h:54-66), read as
+        // TF_SF.cpp:33-89 and 899-910 read them (constants of Constants.h)
+        const double pi = 3.1415926535897932384626433832795, C = 2.99792458E8;
+        std::ifstream in(fileName);
+        if (!in) throw Error(std::string("TF_SF: cannot open ") + fileName);
+        std::string line;
+        auto next = [&]() {
+            std::getline(in, line);
+            return std::istringstream(line);
+        };
+        next() >> lambda;
+        next() >> dlambda;
+        {
+            std::istringstream is = next();
+            is >> w0 >> z0;
+        }
+        {
+            std::istringstream is = next();
+            is >> xs >> ys >> zs;
+        }
+        {
+            std::istringstream is = next();
+            is >> theta_inc >> phi_inc;
+        }
+        {
+            std::istringstream is = next();
+            is >> E0 >> psi_E;
+        }
+        {
+            std::istringstream is = next();
+            is >> nTFSFx1 >> nTFSFx2;
+        }
+        {
+            std::istringstream is = next();
+            is >> nTFSFy1 >> nTFSFy2;
+        }
+        {
+            std::istringstream is = next();
+            is >> nTFSFz1 >> nTFSFz2;
+        }
+        next() >> Nt;
+        next() >> cwmode;
+        theta_inc = theta_inc * pi / 180.0;
+        phi_inc = phi_inc * pi / 180.0;
+        psi_E = psi_E * pi / 180.0;
+        k_num = 2.0 * pi / lambda;
+        zR = pi * w0 * w0 / lambda;
+        if (sS.dx > 0 && sS.dy > 0 && sS.dz > 0) {
+            i0 = (int)(xs / sS.dx);
+            j0 = (int)(ys / sS.dy);
+            k0 = (int)(zs / sS.dz);  // left unset by the reference (F4); the engine uses zs/dz
+        }
+        kx = std::sin(theta_inc) * std::cos(phi_inc);
+        ky = std::sin(theta_inc) * std::sin(phi_inc);
+        kz = std::cos(theta_inc);
+        kabs = std::sqrt(kx * kx + ky * ky + kz * kz);
+        nu = C / lambda;
+        T = 1.0 / nu;
+        if (sS.dt > 0) nT = (int)(T / sS.dt);
     }
+
+    // the reference's public parameters; angles in radians.  z0 is line 3's
+    // beam-waist position (the reference overwrites it with zs/dz, F4)
+    double theta_inc = 0, phi_inc = 0, psi_E = 0, E0 = 0;
+    int nTFSFx1 = 0, nTFSFx2 = 0, nTFSFy1 = 0, nTFSFy2 = 0, nTFSFz1 = 0, nTFSFz2 = 0;
+    double kx = 0, ky = 0, kz = 0, kabs = 0;
+    double lambda = 0, dlambda = 0, nu = 0, T = 0;
+    int nT = 0, Nt = 0;
+    double k_num = 0, w0 = 0, z0 = 0, zR = 0;
+    double xs = 0, ys = 0, zs = 0;
+    int i0 = 0, j0 = 0, k0 = 0;
+    bool cwmode = false;
     void UpdateEField(int timestep, Field &, Structure &)
     {
         ctx->device([timestep](cfd_sim *s) { return cfd_tfsf_e(s, timestep); }, "TF_SF::UpdateEField");

@@ -110,6 +110,58 @@ int main(int, char **argv)
     assert np.array_equal(decode_png(str(out / "line_structure.png")), eng.image_materials(ids))
 
 
+def test_tfsf_public_parameters(tmp_path):
+    """The facade TF_SF exposes the reference's public parameters (main.cpp
+    prints tfsf.lambda and tfsf.nu and loops n < tfsf.Nt), read and derived
+    as TF_SF.cpp does: angles to radians, nu = c / lambda, T = 1 / nu,
+    nT = (int)(T / dt), i0 = (int)(xs / dx)."""
+    if not os.path.exists(engine_build.LIB):
+        pytest.skip("libcpsfdtd.so is not built")
+    case = cases.tiny(str(tmp_path / "case"))
+    src = tmp_path / "tf.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "cpsfdtd.hpp"
+using namespace cpsfdtd;
+int main(int, char **argv)
+{
+    const std::string d = argv[1];
+    Structure S((d + "/material.txt").c_str());
+    S.generate_mesh((d + "/structure.txt").c_str());
+    TF_SF tfsf(S, (d + "/TF_SF.txt").c_str());
+    int steps = 0;
+    for (int n = 1; n < tfsf.Nt; ++n) ++steps;  // main.cpp:267
+    std::printf("%.17g %.17g %.17g %.17g %.17g %.17g %.17g %d %d %d %d %d %d %d %d %.17g %.17g\n", tfsf.lambda,
+                tfsf.nu, tfsf.T, tfsf.theta_inc, tfsf.phi_inc, tfsf.psi_E, tfsf.E0, tfsf.Nt, tfsf.nT, tfsf.i0,
+                tfsf.j0, tfsf.nTFSFx1, tfsf.nTFSFz2, steps, (int)tfsf.cwmode, tfsf.w0, tfsf.kz);
+    return 0;
+}
+''')
+    exe = tmp_path / "tf"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L" + engine_build.LIBDIR, "-lcpsfdtd", "-Wl,-rpath," + engine_build.LIBDIR], check=True)
+    v = subprocess.run([str(exe), case.dir], check=True, capture_output=True, text=True, timeout=60).stdout.split()
+    lines = open(case.files["tfsf"]).read().splitlines()
+    lam = float(lines[0].split()[0])
+    xs = float(lines[3].split()[0])
+    th, ph = (float(x) for x in lines[4].split()[:2])
+    e0, psi = (float(x) for x in lines[5].split()[:2])
+    nt = int(lines[9].split()[0])
+    dt = float(subprocess.run([DRIVER] + ["--material", case.files["material"], "--structure",
+                                          case.files["structure"], "--mesh-only", "--out", str(tmp_path)],
+                              check=True, capture_output=True, text=True).stdout.split()[6]) \
+        if os.path.exists(DRIVER) else None
+    pi, c = 3.1415926535897932384626433832795, 2.99792458E8
+    nu = c / lam
+    assert float(v[0]) == lam and float(v[1]) == nu and float(v[2]) == 1.0 / nu
+    assert float(v[3]) == th * pi / 180.0 and float(v[4]) == ph * pi / 180.0 and float(v[5]) == psi * pi / 180.0
+    assert float(v[6]) == e0 and int(v[7]) == nt == 40 and int(v[13]) == nt - 1
+    if dt is not None:
+        assert int(v[8]) == int((1.0 / nu) / dt)
+    assert int(v[9]) == int(xs / 10e-9) and int(v[11]) == 6 and int(v[12]) == 6
+    assert float(v[16]) == np.cos(th * pi / 180.0)
+
+
 def test_structure_mesh_matches_oracle(tmp_path, driver, oracle_lib):
     """Structure(material) + generate_mesh(structure) through the facade, no
     device: dimensions, time step and the ID map equal the oracle's."""
