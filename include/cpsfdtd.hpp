@@ -35,7 +35,11 @@
  *  - Errors throw cpsfdtd::Error with the library's message (the reference
  *    checks nothing).
  *  - Ports have no trailing unset point (the reference's Num_of_Port_Points
- *    counts one point it never sets, Ports.cpp:74,238,259,280).
+ *    counts one point it never sets, Ports.cpp:74,238,259,280); their text
+ *    exports are otherwise the reference's, byte for byte.  Point lists,
+ *    Export_Port and Export_ID work before the first update without
+ *    starting the simulation.
+ *  - OutImage writes its PNG files but opens no display window.
  *  - Options the reference hard-codes or lacks (precision, number of steps,
  *    defect switches, periodic axes, point sources, port DFT, keeping every
  *    port sample) are set on the Structure before the first update.
