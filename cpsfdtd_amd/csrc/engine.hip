@@ -122,6 +122,33 @@ void launch_full(hipStream_t s, bool h, bool cs, bool nt, dim3 grid, const Dev<R
 #undef CFD_FULL
 }
 
+// Pole counts the special pass has a fixed-count instantiation for (the
+// workloads' media: one Drude and two Lorentz poles, one Drude pole, or no
+// Drude-Lorentz cells at all); any other counts take the generic form.
+// poles = {nD, nL} of every instance with Drude-Lorentz cells, or {0, 0}
+// when none has any; {-1, -1} when they differ.
+template <class R, bool B>
+void launch_special(hipStream_t s, dim3 grid, const Dev<R> &d0, const Dev<R> *devs, const long long *cells, int n,
+                    const SpecialList *lists, const int (&poles)[2])
+{
+#define CFD_SPECIAL(ND, NL) hipLaunchKernelGGL((k_special_e<R, B, ND, NL>), grid, dim3(256), 0, s, d0, devs, cells, n, lists)
+    if (poles[0] == 1 && poles[1] == 2) CFD_SPECIAL(1, 2);
+    else if (poles[0] == 1 && poles[1] == 0) CFD_SPECIAL(1, 0);
+    else if (poles[0] == 0 && poles[1] == 0) CFD_SPECIAL(0, 0);
+    else CFD_SPECIAL(-1, -1);
+#undef CFD_SPECIAL
+}
+
+// fold one instance into the pole counts above (start from {0, 0, 0}: the
+// third entry records whether any instance had Drude-Lorentz cells)
+inline void merge_poles(int (&poles)[3], long long ndl, int nD, int nL)
+{
+    if (!ndl || poles[0] < 0) return;
+    if (nD + nL == 0) { poles[0] = poles[1] = -1; return; }  // no instantiation skips the ADE for them
+    if (!poles[2]) { poles[0] = nD; poles[1] = nL; poles[2] = 1; }
+    else if (poles[0] != nD || poles[1] != nL) poles[0] = poles[1] = -1;
+}
+
 class SimBase {
 public:
     virtual ~SimBase() = default;
@@ -455,6 +482,7 @@ public:
     bool leanH_cs = false, leanE_cs = false;
     long long *special = nullptr;
     int nspecial = 0;
+    int spoles[2] = {-1, -1};  // special pass instantiation (launch_special)
     bool nt_fields = false;  // streamed own-cell field traffic (see stream_fields)
 
     static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1, int tw) {
@@ -547,6 +575,12 @@ public:
         const long long mask = (1LL << kSpecialPml) - 1;
         std::sort(sp.begin(), sp.end(), [&](long long a, long long b) { return (a & mask) < (b & mask); });
         nspecial = (int)sp.size();
+        {
+            int pl[3] = {0, 0, 0};
+            merge_poles(pl, d.ndl, d.nD, d.nL);
+            spoles[0] = pl[0];
+            spoles[1] = pl[1];
+        }
         if (nspecial) special = upload<long long>(sp);
         HIP_OK(hipStreamSynchronize(stream));
     }
@@ -568,7 +602,8 @@ public:
             HIP_OK(hipGetLastError());
             if (!h && nspecial) {
                 timer.run(stream, "special_e", [&] {
-                    hipLaunchKernelGGL((k_special_e<R>), dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, nullptr, special, nspecial, nullptr);
+                    launch_special<R, false>(stream, dim3((nspecial + 255) / 256), d, nullptr, special, nspecial, nullptr,
+                                             spoles);
                 });
                 HIP_OK(hipGetLastError());
             }
@@ -605,7 +640,8 @@ public:
         }
         if (!h && nspecial) {
             timer.run(stream, "special_e", [&] {
-                hipLaunchKernelGGL((k_special_e<R>), dim3((nspecial + 255) / 256), dim3(256), 0, stream, d, nullptr, special, nspecial, nullptr);
+                launch_special<R, false>(stream, dim3((nspecial + 255) / 256), d, nullptr, special, nspecial, nullptr,
+                                         spoles);
             });
             HIP_OK(hipGetLastError());
         }
@@ -864,6 +900,7 @@ public:
     Dev<R> *devs = nullptr;
     SpecialList *lists = nullptr;
     int max_special = 0;
+    int special_poles[2] = {-1, -1};  // one special pass instantiation for every instance
     PortDesc<R> *ports_all = nullptr;  // [instance][port]
     int nports = 0;
     int dft_pending = 0, dft_first = 0;
@@ -883,12 +920,16 @@ public:
             std::vector<Dev<R>> dv;
             std::vector<SpecialList> sl;
             std::vector<PortDesc<R>> pa;
+            int pl[3] = {0, 0, 0};
             for (auto &sp : sims) {
                 dv.push_back(sp->d);
                 sl.push_back(SpecialList{sp->special, sp->nspecial, 0});
                 max_special = std::max(max_special, sp->nspecial);
                 pa.insert(pa.end(), sp->dports.begin(), sp->dports.end());
+                merge_poles(pl, sp->d.ndl, sp->d.nD, sp->d.nL);
             }
+            special_poles[0] = pl[0];
+            special_poles[1] = pl[1];
             nports = (int)sims[0]->dports.size();
             devs = upload(dv);
             lists = upload(sl);
@@ -963,8 +1004,8 @@ public:
             HIP_OK(hipGetLastError());
             if (max_special) {
                 timer.run(stream, "special_e", [&] {
-                    hipLaunchKernelGGL((k_special_e<R, true>), dim3((max_special + 255) / 256, ni), dim3(256), 0, stream, d0, devs,
-                                       nullptr, 0, lists);
+                    launch_special<R, true>(stream, dim3((max_special + 255) / 256, ni), d0, devs, nullptr, 0, lists,
+                                            special_poles);
                 });
                 HIP_OK(hipGetLastError());
             }

@@ -529,6 +529,148 @@ __device__ __forceinline__ void e_full(const Dev<R> &d, int i, int j, int k, uns
     st_once<false>(&d.F[2][p], E[2]);
 }
 
+// e_full with the pole counts fixed at compile time (ND Drude, NL Lorentz
+// poles): the same operations in the same order as e_cell(E_ALL), but every
+// state load of the cell (ADE currents and E^{n-1}, or the gain polarisation
+// and populations) is issued before the first state store.  In the generic
+// form each component's state is stored before the next component's is
+// loaded, an order the compiler must keep (same arrays), so those loads went
+// out one memory round trip at a time.
+template <class R, int ND, int NL>
+__device__ __forceinline__ void e_special(const Dev<R> &d, int i, int j, int k, unsigned pm)
+{
+    const long long p = cell_index(d.px, d.py, i, j, k);
+    const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
+    const long long pjm = (j == 0) ? p + (long long)(d.Ny - 1) * d.px : p - d.px;
+    const long long pkm = p - d.slab;
+    const int id = d.id[p];
+    const int s = d.aux[p];  // state slot of this Drude-Lorentz or gain cell
+    R E[3] = {d.F[0][p], d.F[1][p], d.F[2][p]};
+    const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
+    const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
+    const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
+    const R sxz = d.st[S_cExz][j], sxy = d.st[S_cExy][k], syx = d.st[S_cEyx][k];
+    const R syz = d.st[S_cEyz][i], szy = d.st[S_cEzy][i], szx = d.st[S_cEzx][j];
+    SlabTerm<R> T[3];
+    gather_slabs(d, 0, pm, i, j, k, T);
+    const bool gain = d.ngain && id == 5;
+    const bool dl = (ND + NL > 0) && d.ndl && id == 2;
+
+    // every state load first
+    constexpr int nd = ND > 0 ? ND : 1, nl = NL > 0 ? NL : 1;
+    R jd[3][nd], jl[3][nl], jl1[3][nl], e1[3];
+    const long long ndl = d.ndl, vs = 3 * ndl;
+    if (dl) {
+        for (int c = 0; c < 3; c++) {
+            const R *JD = d.dlst + (long long)c * ndl + s;  // variable v at JD + v*3*ndl
+            const R *JL = JD + (long long)ND * vs;
+            const R *JL1 = JL + (long long)NL * vs;
+            for (int q = 0; q < ND; q++) jd[c][q] = JD[q * vs];
+            for (int q = 0; q < NL; q++) jl[c][q] = JL[q * vs];
+            for (int q = 0; q < NL; q++) jl1[c][q] = JL1[q * vs];
+            e1[c] = JL1[(long long)NL * vs];
+        }
+    }
+    const long long ng = d.ngain;
+    R P[3] = {0, 0, 0}, P1[3] = {0, 0, 0};
+    double N[4] = {0, 0, 0, 0};
+    if (gain) {
+        for (int c = 0; c < 3; c++) {
+            P[c] = d.gP[(long long)c * ng + s];
+            P1[c] = d.gP1[(long long)c * ng + s];
+        }
+        for (int q = 0; q < 4; q++) N[q] = d.gN[(long long)q * ng + s];
+    }
+
+    const R En[3] = {E[0], E[1], E[2]};  // E^n, kept for the gain populations
+    R Pnew[3] = {0, 0, 0};
+    if (gain) {  // polarisation P^{n+1} from E^n
+        const R dNr = (R)(N[1] - N[2]);
+        for (int c = 0; c < 3; c++) {
+            const long long o = (long long)c * ng + s;
+            Pnew[c] = d.CPa1 * P[c] + d.CPa2 * P1[c] + d.CPa3 * dNr * E[c];
+            d.gP[o] = Pnew[c];
+            d.gP1[o] = P[c];
+        }
+    }
+    {  // basic update: ID 2 left to the ADE below
+        const R aE = d.aE[id], bE = d.bE[id];
+        const R u[3] = {aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy,
+                        aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz,
+                        aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx};
+        for (int c = 0; c < 3; c++) E[c] = id == 4 ? (R)0 : id == 2 ? E[c] : u[c];
+    }
+    if (dl) {
+        const R dA[3] = {hz - hz_jm, hx - hx_km, hy - hy_im};
+        const R cA[3] = {sxz, syx, szy};
+        const R dB[3] = {hy - hy_km, hz - hz_im, hx - hx_jm};
+        const R cB[3] = {sxy, syz, szx};
+        for (int c = 0; c < 3; c++) {
+            R *JD = d.dlst + (long long)c * ndl + s;
+            R *JL = JD + (long long)ND * vs;
+            R *JL1 = JL + (long long)NL * vs;
+            R *E1 = JL1 + (long long)NL * vs;
+            const R temp = E[c];
+            R t1 = 0, t2 = 0;
+            for (int q = 0; q < ND; q++) t1 = t1 + d.C4[q] * jd[c][q];
+            for (int q = 0; q < NL; q++) t2 = t2 + d.C5[q] * jl[c][q] + d.C6[q] * jl1[c][q];
+            const R en = d.C1 * E[c] + d.C2 * e1[c] + d.C3 * dA[c] * cA[c] - d.C3 * dB[c] * cB[c] - t1 - t2;
+            E[c] = en;
+            for (int q = 0; q < ND; q++) JD[q * vs] = d.kap[q] * jd[c][q] + d.beta[q] * (en + temp);
+            for (int q = 0; q < NL; q++) {
+                JL[q * vs] = d.alpha[q] * jl[c][q] + d.xi[q] * jl1[c][q] + d.gam[q] * (en - e1[c]) / (R)2.0 / d.dt;
+                JL1[q * vs] = jl[c][q];
+            }
+            *E1 = temp;
+        }
+    }
+    if (gain) {  // E -= (bE/dt) dP, then the populations (FP64)
+        const R coef = d.bE[id] / d.dt;
+        double dot = 0.0;
+        for (int c = 0; c < 3; c++) {
+            const R dP = Pnew[c] - P[c];
+            E[c] = E[c] - coef * dP;
+            dot = dot + (double)(E[c] + En[c]) * (double)dP;
+        }
+        const double Rs = dot / (2.0 * d.g_dt) / d.g_hw;
+        const double h32 = d.g_h32, h21 = d.g_h21, h10 = d.g_h10, dt = d.g_dt;
+        const double n3 = (N[3] * (1.0 - h32) + d.g_dtWp * N[0]) / (1.0 + h32);
+        const double n2 = (N[2] * (1.0 - h21) + h32 * (N[3] + n3) + dt * Rs) / (1.0 + h21);
+        const double n1 = (N[1] * (1.0 - h10) + h21 * (N[2] + n2) - dt * Rs) / (1.0 + h10);
+        d.gN[3 * ng + s] = n3;
+        d.gN[2 * ng + s] = n2;
+        d.gN[ng + s] = n1;
+        d.gN[s] = d.g_Ntot - n1 - n2 - n3;
+    }
+    if (pm) {  // CPML slab terms, the special pass's cells keep them here
+        const R cE = d.cpmlE;
+        if (T[0].side >= 0) {
+            const R a1 = T[0].b * T[0].q1 + T[0].c * (hz - hz_im);
+            const R a2 = T[0].b * T[0].q2 + T[0].c * (hy - hy_im);
+            slab_store(d, 0, 0, T[0], a1, a2);
+            E[1] = E[1] - a1 * cE;
+            E[2] = E[2] + a2 * cE;
+        }
+        if (T[1].side >= 0) {
+            const R a1 = T[1].b * T[1].q1 + T[1].c * (hx - hx_jm);
+            const R a2 = T[1].b * T[1].q2 + T[1].c * (hz - hz_jm);
+            slab_store(d, 0, 1, T[1], a1, a2);
+            E[2] = E[2] - a1 * cE;
+            E[0] = E[0] + a2 * cE;
+        }
+        if (T[2].side >= 0) {
+            const R a1 = T[2].b * T[2].q1 + T[2].c * (hy - hy_km);
+            const R a2 = T[2].b * T[2].q2 + T[2].c * (hx - hx_km);
+            slab_store(d, 0, 2, T[2], a1, a2);
+            E[0] = E[0] - a1 * cE;
+            E[1] = E[1] + a2 * cE;
+        }
+    }
+    d.F[0][p] = E[0];
+    d.F[1][p] = E[1];
+    d.F[2][p] = E[2];
+}
+
 // CPML shell, E: stencil and psi only.  Drude-Lorentz and gain cells of the
 // shell belong to the special pass, which also runs their psi updates.
 template <class R, bool NT>
@@ -671,7 +813,9 @@ struct SpecialList {
     const long long *cells;
     int n, pad;
 };
-template <class R, bool B = false>
+// ND, NL >= 0: every instance's Drude-Lorentz cells have exactly ND Drude and
+// NL Lorentz poles (e_special); -1: any counts (e_full)
+template <class R, bool B = false, int ND = -1, int NL = -1>
 __global__ __launch_bounds__(256) void k_special_e(Dev<R> d0, const Dev<R> *devs, const long long *cells0, int n0,
                                                    const SpecialList *lists)
 {
@@ -685,7 +829,9 @@ __global__ __launch_bounds__(256) void k_special_e(Dev<R> d0, const Dev<R> *devs
     const int k = (int)(p / d.slab);
     const long long r = p - (long long)k * d.slab;
     const int j = (int)(r / d.px), i = (int)(r - (long long)j * d.px);
-    e_full(d, i, j, k, (v >> kSpecialPml) & 1 ? 7u : 0u);
+    const unsigned pm = (v >> kSpecialPml) & 1 ? 7u : 0u;
+    if constexpr (ND < 0) e_full(d, i, j, k, pm);
+    else e_special<R, ND, NL>(d, i, j, k, pm);
 }
 
 // TF_SF.cpp:966-986
