@@ -15,7 +15,7 @@ REF_RESOURCES = "/root/reference/resourses"
 
 
 def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZEROHEAP | SRC_REF_PULSE,
-          zs=0.0, z0=0.0, theta=30.0, phi=45.0, psi=90.0, src=(12, 14, 16), dl=False, gain=False,
+          zs=0.0, z0=0.0, theta=30.0, phi=45.0, psi=90.0, src=(12, 14, 16), dl=False, gain=False, dl_medium=None,
           record=True, freqs=(), stride=1, periodic=0):
     """Every shape painter, CPML on all faces, an oblique Gaussian-beam TF/SF,
     a soft Ez source, a plane and a sphere port."""
@@ -48,8 +48,8 @@ def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZER
     w.source(500e-9, 5e-15, 10e-15)
     if src:
         w.point_source(*src, comp=2, amp=3.0)
-    if dl:
-        w.drude_lorentz(3.7, [(1.37e16, 2.7e13)], [(4.2e15, 1e15, 1.0), (6.1e15, 2e15, 0.5)])
+    if dl:  # dl_medium: (eps_inf, [(wp, gamma)], [(w0, delta, deps)]), default 1 Drude + 2 Lorentz poles
+        w.drude_lorentz(*(dl_medium or (3.7, [(1.37e16, 2.7e13)], [(4.2e15, 1e15, 1.0), (6.1e15, 2e15, 0.5)])))
     if gain:
         w.gain(600e-9, 1e14, 1e-13, 1e-10, 1e-13, 3.3e24, 1e9)
     w.plane_port_cells("pp.txt", "z", n[2] // 2, 2, n[0] - 3, 3, n[1] - 6, d)

@@ -93,6 +93,22 @@ def test_dl_gain_f64_bit_exact(tmp_path):
     assert np.array_equal(e.gain_pop(), o.gain_pop())
 
 
+@pytest.mark.parametrize("poles", ["2d1l", "0d1l"])
+def test_dl_other_pole_counts_f64_bit_exact(tmp_path, poles):
+    """Pole counts without a fixed-count special-pass instantiation (two
+    Drude and one Lorentz pole; one Lorentz pole alone) take the generic
+    cell update; bit-exact against the oracle with dispersive and gain cells
+    in the CPML corners."""
+    medium = {"2d1l": (2.5, [(1.37e16, 2.7e13), (4.0e15, 5e13)], [(5.0e15, 1e15, 0.8)]),
+              "0d1l": (2.0, [], [(4.2e15, 1e15, 1.5)])}[poles]
+    case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED, dl_medium=medium)
+    print("dl poles", poles, flush=True)
+    e, o = run_both(case, 40)
+    assert e.n_dl > 0
+    assert_fields_equal(e, o, "dl " + poles)
+    assert np.array_equal(e.gain_pop(), o.gain_pop())
+
+
 def test_dl_gain_module_f64_bit_exact(tmp_path):
     case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
     e, o = run_both(case, 30, module=True)
