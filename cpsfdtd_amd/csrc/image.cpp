@@ -22,9 +22,11 @@ int fail(const std::string &msg)
     return -1;
 }
 
-// Pixel maps larger than this per side are rejected (PNG allows 2^31 - 1;
-// the limit keeps width * height * 3 well inside size_t and int).
-constexpr int kMaxSide = 1 << 15;
+// Pixel maps larger than this per side are rejected: at 16384 x 16384 RGB
+// the image data (805 MB, plus deflate's worst-case growth) stays inside one
+// PNG chunk, whose length must be below 2^31, and inside zlib's 32-bit CRC
+// length.
+constexpr int kMaxSide = 1 << 14;
 
 bool bad_dims(int nx, int ny) { return nx <= 0 || ny <= 0 || nx > kMaxSide || ny > kMaxSide; }
 

@@ -99,6 +99,8 @@ def test_png_round_trip(tmp_path):
         assert np.array_equal(decode_png(p), rgb)
     with pytest.raises(eng.EngineError, match="cannot open"):
         eng.write_png(str(tmp_path / "no" / "such" / "dir.png"), rgb)
+    with pytest.raises(eng.EngineError, match="bad image size"):  # a side above 16384
+        eng.write_png(str(tmp_path / "wide.png"), np.zeros((1, 16385, 3), np.uint8))
 
 
 def test_outimage_facade(tmp_path):
