@@ -114,6 +114,7 @@ SIGNATURES = {
     "cfd_image_materials": ([C.POINTER(C.c_short), C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_write_png": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_last_error": ([], C.c_char_p),
+    "cfd_config_check": ([C.POINTER(_Config)], C.c_int),
 }
 
 _lib = None

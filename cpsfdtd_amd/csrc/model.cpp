@@ -636,12 +636,30 @@ void cell_lists(Model &m) {
 
 }  // namespace
 
+void check_config(const cfd_config &cfg) {
+    if (!cfg.material || !cfg.structure) throw Error("material and structure files are required");
+    if (cfg.precision != CFD_F32 && cfg.precision != CFD_F64) throw Error("precision must be 32 or 64");
+    if (cfg.nt < 0) throw Error("nt must not be negative");
+    if (cfg.n_ports < 0 || cfg.n_src < 0 || cfg.n_freq < 0) throw Error("negative port, source or frequency count");
+    if (cfg.n_ports > 0) {
+        if (!cfg.port_files) throw Error("n_ports > 0 without port_files");
+        for (int q = 0; q < cfg.n_ports; q++) {
+            if (!cfg.port_files[q]) throw Error("port " + std::to_string(q) + ": no file");
+            if (cfg.port_kinds && cfg.port_kinds[q] != CFD_PORT_PLANE && cfg.port_kinds[q] != CFD_PORT_SPHERE)
+                throw Error("port " + std::to_string(q) + ": kind must be CFD_PORT_PLANE or CFD_PORT_SPHERE");
+        }
+    }
+    if (cfg.n_src > 0 && !cfg.src_ijkc) throw Error("n_src > 0 without src_ijkc");
+    if (cfg.n_freq > 0 && !cfg.freqs) throw Error("n_freq > 0 without freqs");
+    if (cfg.periodic & ~3) throw Error("periodic takes bits 1 (x) and 2 (y) only");
+}
+
 Model build_model(const cfd_config &cfg) {
+    check_config(cfg);
     Model m;
     m.flags = cfg.flags;
     m.pbx = cfg.periodic & 1;
     m.pby = (cfg.periodic >> 1) & 1;
-    if (!cfg.material || !cfg.structure) throw Error("material and structure files are required");
     load_structure(m, cfg.material, cfg.structure);
     if (cfg.pml) load_cpml(m, cfg.pml);
     m.nt = cfg.nt;

@@ -100,6 +100,10 @@ struct Model {
 };
 
 // Builds the model from a configuration; throws cfd::Error on bad input.
+// The fields of a configuration read before any file is opened: required
+// paths, each count against its array, port kinds, precision, periodic axes.
+// Throws Error naming the first problem.
+void check_config(const cfd_config &cfg);
 Model build_model(const cfd_config &cfg);
 
 }  // namespace cfd

@@ -57,6 +57,18 @@ int cfd_model_build(const cfd_config *cfg, cfd_model **out)
 
 const char *cfd_model_last_error(void) { return g_model_error.c_str(); }
 
+int cfd_config_check(const cfd_config *cfg)
+{
+    if (!cfg) { g_model_error = "null configuration"; return -1; }
+    try {
+        cfd::check_config(*cfg);
+        return 0;
+    } catch (const std::exception &e) {
+        g_model_error = e.what();
+    }
+    return -1;
+}
+
 int cfd_model_free(cfd_model *m)
 {
     delete m;

@@ -37,6 +37,8 @@ struct ConfigCopy {
     std::vector<double> amp, phase, freqs;
     cfd_config cfg{};
 
+    // c has passed cfd_config_check; port_kinds, src_amp and src_phase are
+    // optional (null: plane ports, amplitude 1, phase 0) and stay null here
     explicit ConfigCopy(const cfd_config &c) : cfg(c) {
         const char *in[7] = {c.material, c.structure, c.pml, c.tfsf, c.dl, c.gain, c.source};
         std::string *out[7] = {&material, &structure, &pml, &tfsf, &dl, &gain, &source};
@@ -44,12 +46,12 @@ struct ConfigCopy {
             if (in[i]) { *out[i] = in[i]; has[i] = true; }
         for (int q = 0; q < c.n_ports; q++) {
             ports.push_back(c.port_files[q]);
-            kinds.push_back(c.port_kinds[q]);
+            if (c.port_kinds) kinds.push_back(c.port_kinds[q]);
         }
         if (c.n_src > 0) {
             src_ijkc.assign(c.src_ijkc, c.src_ijkc + 4 * c.n_src);
-            amp.assign(c.src_amp, c.src_amp + c.n_src);
-            phase.assign(c.src_phase, c.src_phase + c.n_src);
+            if (c.src_amp) amp.assign(c.src_amp, c.src_amp + c.n_src);
+            if (c.src_phase) phase.assign(c.src_phase, c.src_phase + c.n_src);
         }
         if (c.n_freq > 0) freqs.assign(c.freqs, c.freqs + c.n_freq);
     }
@@ -360,6 +362,10 @@ int cfd_server_destroy(cfd_server *s)
 int cfd_server_submit(cfd_server *s, const cfd_config *cfg, long long *id)
 {
     if (!s || !cfg || !id) { g_server_error = "null argument"; return -1; }
+    if (cfd_config_check(cfg) < 0) {  // a malformed request is refused here, before it is copied
+        g_server_error = std::string("request rejected: ") + cfd_model_last_error();
+        return -1;
+    }
     auto r = std::make_unique<Request>();
     r->cfg = std::make_unique<ConfigCopy>(*cfg);
     std::lock_guard<std::mutex> lk(s->mu);

@@ -189,6 +189,12 @@ int cfd_model_build(const cfd_config *cfg, cfd_model **out);
 int cfd_create_from_model(cfd_model *model, int precision, cfd_sim **out);
 int cfd_model_free(cfd_model *m);
 const char *cfd_model_last_error(void);
+/* A configuration's own fields, checked before any file is read (required
+ * paths, counts against their arrays, port kinds, precision, periodic bits):
+ * 0, or -1 with the message in cfd_model_last_error.  cfd_create,
+ * cfd_model_build and cfd_server_submit apply the same check.  No reference
+ * counterpart (the reference reads its inputs unchecked). */
+int cfd_config_check(const cfd_config *cfg);
 int cfd_model_dims(cfd_model *m, int *nxyz_nt, double *d);
 int cfd_model_id(cfd_model *m, int16_t *host);                      /* Structure::ID */
 /* named FP64 table; returns its length (call with out = NULL to size).

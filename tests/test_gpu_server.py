@@ -132,3 +132,23 @@ def test_engine_nonfinite_count(tmp_path):
     e.set_field(1, f)
     assert e.nonfinite() == 2
     e.close()
+
+
+def test_server_rejects_a_malformed_request_at_submit(tmp_path):
+    """A request whose configuration fails the schema check (ports announced
+    without their files) is refused by cfd_server_submit with the reason,
+    before anything is copied or queued; the server goes on serving."""
+    import ctypes as C
+    srv = eng.Server(max_active=2, quantum=50)
+    good = make(tmp_path, 0, SMALL[0][0])
+    cfg, keep = eng.make_config(good, 64)
+    cfg.port_files = C.POINTER(C.c_char_p)()
+    rid = C.c_longlong(-1)
+    L = eng.library()
+    assert L.cfd_server_submit(srv.h, C.byref(cfg), C.byref(rid)) == -1
+    msg = L.cfd_server_last_error().decode()
+    assert "request rejected" in msg and "port_files" in msg, msg
+    assert srv.stats()["queued"] == 0
+    r = srv.submit(good, 64)
+    assert srv.wait(r, 120.0) == eng.REQ_DONE
+    srv.close()

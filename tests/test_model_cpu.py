@@ -2,6 +2,8 @@
 grading, TF/SF correction lists, ports, dispersive/gain cell lists) against the
 FP64 oracle: integer maps and every FP64 table must be bit-identical.  Runs on
 the CPU through cfd_model_* (no device)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -186,3 +188,73 @@ def test_bad_inputs_fail_loudly(tmp_path):
     case.ports.append((case.path("missing_port.txt"), 0))
     with pytest.raises(eng.EngineError, match="cannot open"):
         eng.Model(case)
+
+
+def _check(cfg):
+    """cfd_config_check: (rc, message)."""
+    L = eng.library()
+    rc = L.cfd_config_check(C.byref(cfg))
+    return rc, L.cfd_model_last_error().decode() if rc < 0 else ""
+
+
+def test_config_schema_rejected_before_any_file_is_read(tmp_path):
+    """Each malformed field of a request is refused by cfd_config_check (and
+    by the model build, which applies it first) with a message naming it;
+    the copies and set-up code never see such a configuration."""
+    case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
+    cfg, keep = eng.make_config(case)
+    assert _check(cfg) == (0, "")
+    null_ptr = C.POINTER(C.c_char_p)()
+    bad = [("structure", None, "material and structure"),
+           ("precision", 16, "precision"),
+           ("nt", -1, "nt"),
+           ("n_src", -2, "negative"),
+           ("port_files", null_ptr, "without port_files"),
+           ("src_ijkc", C.POINTER(C.c_int)(), "without src_ijkc"),
+           ("periodic", 4, "periodic")]
+    for field, value, msg in bad:
+        c2, keep2 = eng.make_config(case)
+        setattr(c2, field, value)
+        rc, text = _check(c2)
+        assert rc == -1 and msg in text, (field, text)
+    c2, keep2 = eng.make_config(case)
+    c2.n_freq, c2.freqs = 2, C.POINTER(C.c_double)()
+    assert "without freqs" in _check(c2)[1]
+    c2, keep2 = eng.make_config(case)
+    c2.port_files[1] = None
+    assert "port 1: no file" in _check(c2)[1]
+    c2, keep2 = eng.make_config(case)
+    kinds = (C.c_int * 2)(0, 7)
+    c2.port_kinds = C.cast(kinds, C.POINTER(C.c_int))
+    assert "port 1: kind" in _check(c2)[1]
+    h = C.c_void_p()
+    assert eng.library().cfd_model_build(C.byref(c2), C.byref(h)) == -1  # the build refuses it too
+
+
+def test_config_optional_arrays(tmp_path):
+    """port_kinds, src_amp and src_phase may be null (plane ports, amplitude
+    1, phase 0): such a configuration passes the check and builds, with the
+    same port point lists as the one that gives those values."""
+    case = cases.mixed(str(tmp_path), flags=FIXED)
+    case.ports = [p for p in case.ports if p[1] == 0]  # plane ports only
+    case.sources = [(s[0], s[1], s[2], s[3], 1.0, 0.0) for s in case.sources]
+    assert case.ports and case.sources
+    cfg, keep = eng.make_config(case)
+    nulls, keep2 = eng.make_config(case)
+    nulls.port_kinds = C.POINTER(C.c_int)()
+    nulls.src_amp = C.POINTER(C.c_double)()
+    nulls.src_phase = C.POINTER(C.c_double)()
+    assert _check(nulls) == (0, "")
+    L = eng.library()
+    ha, hb = C.c_void_p(), C.c_void_p()
+    assert L.cfd_model_build(C.byref(cfg), C.byref(ha)) == 0 and L.cfd_model_build(C.byref(nulls), C.byref(hb)) == 0
+    try:
+        for q in range(len(case.ports)):
+            n = L.cfd_model_port(ha, q, None)
+            a, b = np.zeros(3 * n, np.int32), np.zeros(3 * n, np.int32)
+            L.cfd_model_port(ha, q, a.ctypes.data_as(C.POINTER(C.c_int)))
+            L.cfd_model_port(hb, q, b.ctypes.data_as(C.POINTER(C.c_int)))
+            assert np.array_equal(a, b)
+    finally:
+        L.cfd_model_free(ha)
+        L.cfd_model_free(hb)
