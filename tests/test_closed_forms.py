@@ -432,3 +432,36 @@ def test_yee_cavity_modes_and_energy(oracle_lib, tmp_path, eps_r):
         assert abs(f_meas / f_disc - 1) < 1e-5, (p, f_meas / f_disc - 1)
         if p == 15:
             assert abs(f_meas / f_cont - 1) > 1e-2, (p, f_meas / f_cont - 1)
+
+
+def _block_case(directory, dispersive):
+    """A 6x5x4-cell block of eps_r 2.25 in a CPML-bounded grid with a soft
+    source: as a plain medium (material 1) or as the Drude-Lorentz medium
+    (ID 2) with eps_inf = 2.25 and no poles."""
+    w = CaseWriter(str(directory))
+    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (1, 1, 0, 0)])
+    w.grid((24, 22, 20), (10e-9, 10e-9, 10e-9),
+           [Shape(2 if dispersive else 1, "cuboid", x0=120e-9, y0=110e-9, z0=100e-9, a=60e-9, b=50e-9, c=40e-9)])
+    if dispersive:
+        w.drude_lorentz(2.25, [], [])
+    w.cpml(((4, 4), (4, 4), (4, 4)), m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.source(300e-9, 3e-15, 5e-15)
+    w.point_source(8, 9, 10, comp=2, amp=2.0)
+    return w.run(300, flags=FIXED)
+
+
+def test_drude_lorentz_without_poles_is_the_dielectric(oracle_lib, tmp_path):
+    """With no Drude or Lorentz pole the ADE update is the basic update of a
+    lossless dielectric of permittivity eps_inf (C1 = 1, C2 = 0, C3 = bE):
+    the block as ADE cells and as a plain eps_r 2.25 medium give the same
+    fields bit for bit after 300 steps."""
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    plain = oracle_lib.OracleSim(_block_case(tmp_path / "a", False))
+    ade = oracle_lib.OracleSim(_block_case(tmp_path / "b", True))
+    assert ade.dl_ncells() > 0
+    plain.step(300)
+    ade.step(300)
+    for c in range(6):
+        assert np.array_equal(plain.field(c), ade.field(c)), c
+    assert np.abs(plain.field(2)).max() > 1e-6

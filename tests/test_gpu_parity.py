@@ -109,6 +109,22 @@ def test_dl_other_pole_counts_f64_bit_exact(tmp_path, poles):
     assert np.array_equal(e.gain_pop(), o.gain_pop())
 
 
+def test_dl_without_poles_is_the_dielectric_f64(tmp_path, oracle_lib):
+    """Zero Drude and Lorentz poles take the generic special-pass update; it
+    must reduce to the plain dielectric bit for bit (as on the oracle)."""
+    from test_closed_forms import _block_case
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    o = oracle_lib.OracleSim(_block_case(tmp_path / "a", False))
+    o.step(300)
+    print("dl without poles", flush=True)
+    e = eng.Engine(_block_case(tmp_path / "b", True), 64)
+    assert e.n_dl > 0
+    e.step(300)
+    assert_fields_equal(e, o, "ade without poles")
+    e.close()
+
+
 def test_dl_gain_module_f64_bit_exact(tmp_path):
     case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
     e, o = run_both(case, 30, module=True)
