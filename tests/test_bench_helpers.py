@@ -47,3 +47,13 @@ def test_cpu_fifo_model_queue(monkeypatch):
     r = bench.cpu_fifo_model(3, 0.1)
     assert r["makespan_s"] == 54.0
     assert r["p50_s"] == 3.0 and r["p99_s"] == round(float(np.percentile([1.0, 3.0, 4.0], 99)), 1)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """This engine runs on one GPU: --gpus 2 exits with a message (status 2)
+    before touching a device."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr, (r.returncode, r.stderr[-300:])
