@@ -192,9 +192,10 @@ public:
     TfTarget *tfE = nullptr, *tfH = nullptr;
     int ntfE = 0, ntfH = 0, tablen = 0;
     double *tcoef = nullptr, *tdelay = nullptr, *tprof = nullptr, *einc = nullptr, *hinc = nullptr;
-    // sources
-    long long *scell = nullptr;
-    int *scomp = nullptr;
+    // sources, grouped by (cell, component)
+    SourceGroup *sgroup = nullptr;
+    int nsgroup = 0;
+    int *sidx = nullptr;
     double *swave = nullptr;
     // ports
     std::vector<int> dl_slot, gain_slot;  // reference-order cell -> device slot
@@ -363,7 +364,7 @@ public:
         }
         if (m.has_gain) {
             d.ngain = (int)m.gain_cells.size();
-            d.CPa1 = (R)m.gCPa1; d.CPa2 = (R)m.gCPa2; d.CPa3 = (R)m.gCPa3;
+            d.CPa1 = (R)m.gCPa1; d.CPa2 = (R)m.gCPa2; d.g_CPa3 = m.gCPa3;
             d.g_dt = m.dt;
             d.g_hw = kHPlanck / (2.0 * kPi) * m.g_omega;
             d.g_Ntot = m.g_Ntot;
@@ -415,13 +416,25 @@ public:
         }
         // sources
         if (!m.src.empty()) {
-            std::vector<long long> cells;
-            std::vector<int> comps;
-            for (size_t q = 0; q < m.src.size(); q += 4) {
-                cells.push_back(dev_index(m.src[q], m.src[q + 1], m.src[q + 2]));
-                comps.push_back(m.src[q + 3]);
+            const int ns = (int)(m.src.size() / 4);
+            std::vector<std::pair<long long, int>> key(ns);  // (cell * 3 + comp, source)
+            for (int q = 0; q < ns; q++)
+                key[q] = {dev_index(m.src[4 * q], m.src[4 * q + 1], m.src[4 * q + 2]) * 3 + m.src[4 * q + 3], q};
+            std::stable_sort(key.begin(), key.end(),
+                             [](const std::pair<long long, int> &a, const std::pair<long long, int> &b) { return a.first < b.first; });
+            std::vector<SourceGroup> groups;
+            std::vector<int> idx;
+            for (int r = 0; r < ns; r++) {
+                if (r == 0 || key[r].first != key[r - 1].first)
+                    groups.push_back({key[r].first / 3, (int)(key[r].first % 3), r, 0, 0});
+                groups.back().count++;
+                idx.push_back(key[r].second);
             }
-            scell = upload<long long>(cells); scomp = upload<int>(comps); swave = upload<double>(m.src_wave);
+            nsgroup = (int)groups.size();
+            sgroup = alloc<SourceGroup>(groups.size());
+            HIP_OK(hipMemcpyAsync(sgroup, groups.data(), groups.size() * sizeof(SourceGroup), hipMemcpyHostToDevice, stream));
+            sidx = upload<int>(idx);
+            swave = upload<double>(m.src_wave);
         }
         // ports
         if (!m.freqs.empty()) { twE = upload<double>(m.twE); twH = upload<double>(m.twH); }
@@ -668,12 +681,11 @@ public:
         HIP_OK(hipGetLastError());
     }
     void sources(int n) override {
-        const int ns = (int)(m.src.size() / 4);
-        if (!ns) return;
+        if (!nsgroup) return;
         if (!nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
         timer.run(stream, "sources", [&] {
-            hipLaunchKernelGGL(k_sources<R>, dim3((ns + 63) / 64), dim3(64), 0, stream, d, scell, scomp, swave, ns, m.nt + 1,
-                               nbase_cur, n);
+            hipLaunchKernelGGL(k_sources<R>, dim3((nsgroup + 63) / 64), dim3(64), 0, stream, d, sgroup, nsgroup, sidx,
+                               swave, m.nt + 1, nbase_cur, n);
         });
         HIP_OK(hipGetLastError());
     }

@@ -57,7 +57,8 @@ struct Dev {
     R *dlst;  // [(var*3 + comp) * ndl + slot]
     // gain (populations FP64)
     int ngain;
-    R CPa1, CPa2, CPa3;
+    R CPa1, CPa2;
+    double g_CPa3;  // P coupling to the inversion, FP64 (subnormal in FP32)
     double g_dt, g_hw, g_Ntot, g_h32, g_h21, g_h10, g_dtWp;
     R *gP, *gP1, *gEprev;  // [comp * ngain + slot]
     double *gN;            // [level * ngain + slot], N0..N3

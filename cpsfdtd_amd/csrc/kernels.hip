@@ -198,12 +198,14 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
     if (gain) {
         gs = d.aux[p];
         if (parts & E_GAINPOL) {
+            // CPa3 (N1 - N2) formed in FP64 and rounded once: CPa3 alone is
+            // below FP32's smallest normal number (about 7e-39 at 20 nm cells)
             const double dN = d.gN[1LL * d.ngain + gs] - d.gN[2LL * d.ngain + gs];
-            const R dNr = (R)dN;
+            const R cdN = (R)(d.g_CPa3 * dN);
             for (int c = 0; c < 3; c++) {
                 const long long o = (long long)c * d.ngain + gs;
                 const R Pn = d.gP[o];
-                d.gP[o] = d.CPa1 * Pn + d.CPa2 * d.gP1[o] + d.CPa3 * dNr * E[c];
+                d.gP[o] = d.CPa1 * Pn + d.CPa2 * d.gP1[o] + cdN * E[c];
                 d.gP1[o] = Pn;
                 if (!(parts & E_GAINPOP)) d.gEprev[o] = E[c];
             }
@@ -585,10 +587,10 @@ __device__ __forceinline__ void e_special(const Dev<R> &d, int i, int j, int k, 
     const R En[3] = {E[0], E[1], E[2]};  // E^n, kept for the gain populations
     R Pnew[3] = {0, 0, 0};
     if (gain) {  // polarisation P^{n+1} from E^n
-        const R dNr = (R)(N[1] - N[2]);
+        const R cdN = (R)(d.g_CPa3 * (N[1] - N[2]));  // FP64 product, as in e_cell
         for (int c = 0; c < 3; c++) {
             const long long o = (long long)c * ng + s;
-            Pnew[c] = d.CPa1 * P[c] + d.CPa2 * P1[c] + d.CPa3 * dNr * E[c];
+            Pnew[c] = d.CPa1 * P[c] + d.CPa2 * P1[c] + cdN * E[c];
             d.gP[o] = Pnew[c];
             d.gP1[o] = P[c];
         }
@@ -908,16 +910,26 @@ __global__ void k_set_int(int *dst, int v)
     if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
 }
 
-// soft point sources (main.cpp:284-288)
+// Soft point sources (main.cpp:284-288).  Sources are grouped by (cell,
+// component); one thread adds its group's waveforms one after another in
+// source order, as the reference adds sources, so sources sharing a cell
+// neither race nor change the rounding.
+struct SourceGroup {
+    long long cell;
+    int comp, first, count, pad;  // members idx[first .. first + count)
+};
 template <class R>
-__global__ void k_sources(Dev<R> d, const long long *cell, const int *comp, const double *wave, int nsrc, int stride,
-                          const int *nbase, int n)
+__global__ void k_sources(Dev<R> d, const SourceGroup *grp, int ngroups, const int *idx, const double *wave,
+                          int stride, const int *nbase, int n)
 {
     if (nbase) n += *nbase;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nsrc) return;
-    R *F = d.F[comp[q]];
-    F[cell[q]] = F[cell[q]] + (R)wave[(long long)q * stride + n];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const SourceGroup G = grp[g];
+    R *F = d.F[G.comp];
+    R v = F[G.cell];
+    for (int q = G.first; q < G.first + G.count; q++) v = v + (R)wave[(long long)idx[q] * stride + n];
+    F[G.cell] = v;
 }
 
 // One port of a simulation as the port kernels see it.  Points of all ports

@@ -17,6 +17,7 @@ const std::vector<double> *find_table(const cfd::Model &m, const std::string &n,
     if (n == "bH") return &m.bH;
     if (n == "Einc") return &m.Einc;
     if (n == "Hinc") return &m.Hinc;
+    if (n == "src_wave") return &m.src_wave;  // [source][nt + 1]
     if (n.rfind("stretch", 0) == 0) {
         const int t = std::atoi(n.c_str() + 7);
         return (t >= 0 && t < cfd::kNumStretch) ? &m.stretch[t] : nullptr;

@@ -267,9 +267,16 @@ struct cfd_server {
                 if (stop) break;
                 act = active;
             }
-            // every idle request gets its next quantum (asynchronous, own stream)
+            // every idle request gets its next quantum (asynchronous, own stream);
+            // a request retired here belongs to the host thread from then on
+            // (it may be finished and released at once), so the loop below
+            // must not touch it
+            std::vector<Request *> stepping;
             for (Request *r : act) {
-                if (r->inflight) continue;
+                if (r->inflight) {
+                    stepping.push_back(r);
+                    continue;
+                }
                 const int k = std::min(opts.quantum, r->nt - r->done);
                 if (k <= 0 || cfd_step(r->sim, k) < 0) {
                     retire(r, k > 0 ? std::string(cfd_last_error()) : std::string());
@@ -278,10 +285,10 @@ struct cfd_server {
                 r->done += k;
                 r->inflight = true;
                 steps_launched += k;
+                stepping.push_back(r);
             }
             bool any = false;
-            for (Request *r : act) {
-                if (!r->inflight) continue;
+            for (Request *r : stepping) {
                 const int q = cfd_query(r->sim);
                 if (q == 0) continue;
                 any = true;

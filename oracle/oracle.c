@@ -609,6 +609,7 @@ static void cpml_update_H(orc_sim *s)
     int Nx = s->Nx, Ny = s->Ny, Nz = s->Nz;
     double cH = P->cpmlH;
     double sg = (s->flags & ORC_FIX_CPML_HSIGN) ? -1.0 : 1.0;
+    const int flipx = (s->flags & ORC_TEST_FLIP_HX) != 0;
     /* x slabs: Hy += psi(dEz/dx), Hz -= psi(dEy/dx) */
     for (int e = 0; e < 2; e++) {
         int n = P->npml[0][e];
@@ -622,9 +623,14 @@ static void cpml_update_H(orc_sim *s)
                 for (int k = 0; k < Nz; ++k) {
                     size_t p = IDX(s, i, j, k), qi = qidx(s, 0, n, l, i, j, k);
                     q1[qi] = b * q1[qi] + c * (Ez[IDX(s, i + 1, j, k)] - Ez[p]);
-                    Hy[p] = Hy[p] + q1[qi] * cH;
                     q2[qi] = b * q2[qi] + c * (Ey[IDX(s, i + 1, j, k)] - Ey[p]);
-                    Hz[p] = Hz[p] - q2[qi] * cH;
+                    if (flipx) {  /* planted mistake (ORC_TEST_FLIP_HX) */
+                        Hy[p] = Hy[p] - q1[qi] * cH;
+                        Hz[p] = Hz[p] + q2[qi] * cH;
+                    } else {
+                        Hy[p] = Hy[p] + q1[qi] * cH;
+                        Hz[p] = Hz[p] - q2[qi] * cH;
+                    }
                 }
         }
     }
@@ -1461,6 +1467,13 @@ void orc_set_field(orc_sim *s, int comp, const double *in) { memcpy(s->F[comp], 
 int orc_gain_ncells(const orc_sim *s) { return s->gn.ncell; }
 void orc_gain_pop(const orc_sim *s, double *out) { memcpy(out, s->gn.N, sizeof(double) * 4 * (size_t)s->gn.ncell); }
 int orc_dl_ncells(const orc_sim *s) { return s->dl.ncell; }
+
+int orc_source_wave(const orc_sim *s, int q, double *out)
+{
+    if (q < 0 || q >= s->nsrc) return 0;
+    if (out) memcpy(out, s->swave + (size_t)q * (s->nt + 1), sizeof(double) * (s->nt + 1));
+    return s->nt + 1;
+}
 int orc_dl_coefs(const orc_sim *s, double *o)
 {
     const dl_t *D = &s->dl;

@@ -33,6 +33,10 @@ extern "C" {
  * the field component that supplies it (plus half a cell along the corrected
  * E component's own axis, for the E and the H correction of that pair). */
 #define ORC_FIX_TFSF_YEE    16u
+/* A planted mistake, for tests that must fail on it: the x-slab CPML H terms
+ * (CPML.cpp:618-620) applied with the opposite sign.  No run of the
+ * reference or the engine uses it. */
+#define ORC_TEST_FLIP_HX  (1u << 30)
 
 #define ORC_PORT_PLANE  0
 #define ORC_PORT_SPHERE 1
@@ -105,6 +109,9 @@ void orc_set_field(orc_sim *s, int comp, const double *in);
 int orc_gain_ncells(const orc_sim *s);
 void orc_gain_pop(const orc_sim *s, double *out);
 int orc_dl_ncells(const orc_sim *s);
+/* Point source q's waveform, one value per time index 0..nt (added after the
+ * E update of step n); returns nt + 1, or 0 for no such source. */
+int orc_source_wave(const orc_sim *s, int q, double *out);
 /* DL coefficients: C1 C2 C3 then C4[nD] C5[nL] C6[nL] kappa[nD] beta[nD] alpha[nL] xi[nL] gammaL[nL] */
 int orc_dl_coefs(const orc_sim *s, double *out);
 

@@ -68,6 +68,7 @@ def lib():
             ("orc_get_field", [vp, C.c_int, dp], None), ("orc_set_field", [vp, C.c_int, dp], None),
             ("orc_gain_ncells", [vp], C.c_int), ("orc_gain_pop", [vp, dp], None),
             ("orc_dl_ncells", [vp], C.c_int), ("orc_dl_coefs", [vp, dp], C.c_int),
+            ("orc_source_wave", [vp, C.c_int, dp], C.c_int),
             ("orc_dl_0d", [vp, C.c_int, dp, dp], C.c_int), ("orc_gain_0d", [vp, C.c_int, dp, dp, dp], C.c_int),
         ]:
             f = getattr(L, name)
@@ -240,6 +241,13 @@ class OracleSim:
         n = lib().orc_gain_ncells(self.h)
         a = np.zeros((n, 4), np.float64)
         lib().orc_gain_pop(self.h, _dp(a))
+        return a
+
+    def source_wave(self, q):
+        n = lib().orc_source_wave(self.h, q, None)
+        a = np.zeros(n, np.float64)
+        if n:
+            lib().orc_source_wave(self.h, q, _dp(a))
         return a
 
     def dl_ncells(self):

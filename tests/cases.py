@@ -16,12 +16,15 @@ REF_RESOURCES = "/root/reference/resourses"
 
 def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZEROHEAP | SRC_REF_PULSE,
           zs=0.0, z0=0.0, theta=30.0, phi=45.0, psi=90.0, src=(12, 14, 16), dl=False, gain=False, dl_medium=None,
-          record=True, freqs=(), stride=1, periodic=0):
-    """Every shape painter, CPML on all faces, an oblique Gaussian-beam TF/SF,
-    a soft Ez source, a plane and a sphere port."""
+          record=True, freqs=(), stride=1, periodic=0, cw=0, magnetic=False):
+    """Every shape painter, CPML on all faces, an oblique Gaussian-beam TF/SF
+    (cw=1: the continuous-wave form), a soft Ez source, a plane and a sphere
+    port.  magnetic: media 1 and 3 get mu_r 2 and 1.5, medium 3 a magnetic
+    conductivity of 1e9 ohm/m (sigma_M dt / 2 mu0 mu_r about 0.5 %)."""
     d = (10e-9, 10e-9, 10e-9)
     w = CaseWriter(directory)
-    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (1, 1, 0, 0), (4.0, 1, 0.01, 0), (1, 1, 0, 0), (1.5, 1, 0, 0)])
+    w.materials([(1, 1, 0, 0), (2.25, 2.0 if magnetic else 1, 0, 0), (1, 1, 0, 0),
+                 (4.0, 1.5 if magnetic else 1, 0.01, 1e9 if magnetic else 0), (1, 1, 0, 0), (1.5, 1, 0, 0)])
     sx, sy, sz = n[0] / 40.0, n[1] / 36.0, n[2] / 32.0
     w.grid(n, d, [
         Shape(1, "ellipsoid", x0=200e-9 * sx, y0=180e-9 * sy, z0=160e-9 * sz, a=80e-9, b=70e-9, c=60e-9),
@@ -44,7 +47,7 @@ def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZER
         if zs != 0.0:
             z0 = zs / d[2]  # numerically the value the reference leaves in z0 (TF_SF.cpp:57)
         w.tfsf(500e-9, dlambda_for_fwhm(500e-9, 5e-15), 300e-9, z0, n[0] * 5e-9, n[1] * 5e-9, zs,
-               theta, phi, 1.0, psi, ((box, box), (box, box), (box, box)), nt)
+               theta, phi, 1.0, psi, ((box, box), (box, box), (box, box)), nt, cw=cw)
     w.source(500e-9, 5e-15, 10e-15)
     if src:
         w.point_source(*src, comp=2, amp=3.0)
@@ -57,15 +60,17 @@ def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZER
     return w.run(nt, freqs=freqs, dft_stride=stride, record_ports=record, periodic=periodic, flags=flags)
 
 
-def tiny(directory, nt=40, flags=REF_ZEROHEAP | SRC_REF_PULSE, text_ports=False):
+def tiny(directory, nt=40, flags=REF_ZEROHEAP | SRC_REF_PULSE, text_ports=False, cw=0, magnetic=False):
     """24x22x20 grid with every module the reference has; small enough that
     full field snapshots fit in a fixture.  text_ports adds a 2-D x-normal
     plane port (6 x 4 points) and a partial sphere port, for the text
-    exports."""
+    exports.  cw=1: continuous-wave TF/SF.  magnetic: media 1 and 3 get mu_r
+    2 and 1.5, medium 3 a magnetic conductivity of 1e9 ohm/m."""
     d = (10e-9, 10e-9, 10e-9)
     n = (24, 22, 20)
     w = CaseWriter(directory)
-    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0), (1, 1, 0, 0), (4.0, 1, 0.02, 0), (1, 1, 0, 0)])
+    w.materials([(1, 1, 0, 0), (2.25, 2.0 if magnetic else 1, 0, 0), (1, 1, 0, 0),
+                 (4.0, 1.5 if magnetic else 1, 0.02, 1e9 if magnetic else 0), (1, 1, 0, 0)])
     w.grid(n, d, [
         Shape(1, "ellipsoid", x0=120e-9, y0=110e-9, z0=100e-9, a=40e-9, b=45e-9, c=35e-9),
         Shape(3, "cuboid", x0=60e-9, y0=70e-9, z0=60e-9, a=30e-9, b=40e-9, c=30e-9),
@@ -73,7 +78,7 @@ def tiny(directory, nt=40, flags=REF_ZEROHEAP | SRC_REF_PULSE, text_ports=False)
     ])
     w.cpml(((4, 4), (4, 4), (4, 4)), m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
     w.tfsf(300e-9, dlambda_for_fwhm(300e-9, 3e-15), 200e-9, 0.0, 120e-9, 110e-9, 0.0, 25.0, 35.0, 1.0, 70.0,
-           ((6, 6), (6, 6), (6, 6)), nt)
+           ((6, 6), (6, 6), (6, 6)), nt, cw=cw)
     w.source(300e-9, 3e-15, 5e-15)
     w.point_source(8, 9, 10, comp=2, amp=2.0)
     w.plane_port_cells("line.txt", "z", 10, 1, 22, 11, 11, d)

@@ -2,7 +2,8 @@
 core (oracle/_ref/refdrv, built by `make -C oracle ref` from /root/reference).
 
 Run in a container that has /root/reference:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [shipped] [mixed] [tiny] [text] [extra]
+(no argument: all of them)
 
 Fixtures (all data, no reference source):
   inputs/               the reference's example inputs used here (resourses/*.txt)
@@ -13,6 +14,15 @@ Fixtures (all data, no reference source):
                         oblique TF/SF, plane + sphere port), with k0 = 0 and with
                         a nonzero beam reference height (k0 = zs/dz)
   ref_tiny.npz          40 steps of tests.cases.tiny: full fields and port samples
+  ref_extra.npz         the continuous-wave TF/SF form (TF_SF.cpp:917-926):
+                        Einc/Hinc and incident vectors of tests.cases.mixed(cw=1)
+                        over 600 steps (ramp and steady branches), and SHA-256
+                        digests of the fields and port samples after 300 steps
+                        of tests.cases.tiny(cw=1); media with mu_r != 1 and a
+                        magnetic conductivity (Structure.cpp:352-357): the
+                        coefficient tables of tests.cases.mixed(magnetic=True)
+                        and digests after 100 steps of tests.cases.tiny(
+                        magnetic=True)
   ref_port_text.npz     the text the reference's Ports.cpp Export_* methods write
                         over 40 steps of tests.cases.tiny(text_ports=True) (a
                         line, a 2-D plane and a partial sphere port; plane
@@ -110,15 +120,16 @@ def strip_f11(texts, q, sphere, nt):
     return {"port%d_%s" % (q, k): v for k, v in r.items()}
 
 
-def main():
-    po.build()
-    if not po.ref_available():
-        raise SystemExit("reference driver missing: run make -C oracle ref (needs /root/reference)")
+def digest(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest().encode(),
+                         dtype=np.uint8)
+
+
+def make_shipped():
     inp = os.path.join(HERE, "inputs")
     os.makedirs(inp, exist_ok=True)
     for f in SHIPPED:
         shutil.copy(os.path.join(cases.REF_RESOURCES, f), os.path.join(inp, f))
-
     out = {}
     for tag, sfile, pml in [("structure", "structure.txt", True), ("shi1", "Shi1.txt", False)]:
         d = tempfile.mkdtemp()
@@ -141,6 +152,8 @@ def main():
             out[tag + "_cpml_c"] = r["cpml_c"]
     np.savez_compressed(os.path.join(HERE, "ref_shipped.npz"), **out)
 
+
+def make_mixed():
     out = {}
     # k0set: the TF_SF object's unset k0 is given the value the oracle derives,
     # (int)(zs/dz), so both place the beam reference point identically
@@ -151,6 +164,8 @@ def main():
         tables(tag + "_", r, out)
     np.savez_compressed(os.path.join(HERE, "ref_mixed.npz"), **out)
 
+
+def make_tiny():
     d = tempfile.mkdtemp()
     case = cases.tiny(d)
     F, P = po.ref_run(case, case.nt, src=(8, 9, 10, 2.0))
@@ -160,6 +175,8 @@ def main():
     out["port0"] = P[0]
     np.savez_compressed(os.path.join(HERE, "ref_tiny.npz"), **out)
 
+
+def make_text():
     d = tempfile.mkdtemp()
     case = cases.tiny(d, text_ports=True)
     texts = po.ref_port_text(case, case.nt, src=(8, 9, 10, 2.0))
@@ -168,6 +185,36 @@ def main():
         for name, text in strip_f11(texts, q, kind == 1, case.nt).items():
             out[name] = np.frombuffer(text.encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(HERE, "ref_port_text.npz"), **out)
+
+
+def make_extra():
+    out = {}
+    # nt >= the 5-period ramp (437 steps here): the reference writes its ramp
+    # past Einc/Hinc[Nt] otherwise (TF_SF.cpp:917-921)
+    r = po.ref_tables(cases.mixed(tempfile.mkdtemp(), cw=1, nt=600))
+    for k in ("tfsf_einc", "tfsf_hinc", "tfsf_vec"):
+        out["cw_" + k] = r[k]
+    r = po.ref_tables(cases.mixed(tempfile.mkdtemp(), magnetic=True))
+    out["mag_coef"] = r["coef"]
+    for tag, kw, nt in (("cw", {"cw": 1}, 300), ("mag", {"magnetic": True}, 100)):
+        case = cases.tiny(tempfile.mkdtemp(), nt=nt, **kw)
+        F, P = po.ref_run(case, nt, src=(8, 9, 10, 2.0))
+        out[tag + "_nt"] = np.array([nt])
+        out[tag + "_field_sha256"] = np.array([digest(F[c]) for c in range(6)])
+        out[tag + "_field_l2"] = np.array([float(np.linalg.norm(F[c])) for c in range(6)])
+        out[tag + "_port0_sha256"] = digest(P[0])
+    np.savez_compressed(os.path.join(HERE, "ref_extra.npz"), **out)
+
+
+MAKERS = {"shipped": make_shipped, "mixed": make_mixed, "tiny": make_tiny, "text": make_text, "extra": make_extra}
+
+
+def main():
+    po.build()
+    if not po.ref_available():
+        raise SystemExit("reference driver missing: run make -C oracle ref (needs /root/reference)")
+    for name in sys.argv[1:] or list(MAKERS):
+        MAKERS[name]()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -30,13 +30,19 @@ def test_mt19937_64_seeding():
 
 
 def test_rod_positions():
-    pts = configs.rod_positions(400, 192, 192, 3, 7.0, 1234)
+    """c3's rods: addition at 7 cells then relaxation to the configured 8-cell
+    minimum spacing, seeded and repeatable."""
+    pts = configs.rod_positions(400, 192, 192, 3, 8.0, 1234, rsa_spacing=7.0)
     assert len(pts) == 400
-    assert pts == configs.rod_positions(400, 192, 192, 3, 7.0, 1234)
+    assert pts == configs.rod_positions(400, 192, 192, 3, 8.0, 1234, rsa_spacing=7.0)
     p = np.array(pts)
     assert p.min() >= 4 and p.max() < 188
     d2 = ((p[:, None, :] - p[None, :, :]) ** 2).sum(-1) + np.eye(400) * 1e9
-    assert d2.min() >= 49.0
+    assert d2.min() >= 64.0
+    # relaxation moves rods: the addition-only layout had pairs under 8 cells
+    q = np.array(configs.rod_positions(400, 192, 192, 3, 7.0, 1234))
+    e2 = ((q[:, None, :] - q[None, :, :]) ** 2).sum(-1) + np.eye(400) * 1e9
+    assert 49.0 <= e2.min() < 64.0
 
 
 def model_vs_oracle(case, oracle_lib):

@@ -270,3 +270,60 @@ def test_empty_case_steps_and_stays_zero(tmp_path):
             assert not np.any(e.field(c)), (prec, c)
         assert e.nonfinite() == 0
         e.close()
+
+
+@pytest.mark.parametrize("tag,kw", [("cw", {"cw": 1}), ("mag", {"magnetic": True})])
+def test_tiny_variants_match_reference_fixture(tmp_path, tag, kw):
+    """The reference's own loop with continuous-wave TF/SF and with magnetic
+    media (fixture digests of its fields and port samples), reproduced on the
+    GPU bit for bit."""
+    import hashlib
+    g = dict(np.load(os.path.join(GOLD, "ref_extra.npz"), allow_pickle=False))
+    nt = int(g[tag + "_nt"][0])
+    case = cases.tiny(str(tmp_path), nt=nt, **kw)
+    print("tiny", tag, nt, "steps", flush=True)
+    e = eng.Engine(case, 64)
+    e.step(nt)
+
+    def sha(a):
+        return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    for c in range(6):
+        assert sha(e.field(c).ravel()) == bytes(g[tag + "_field_sha256"][c]).decode(), c
+    assert sha(e.port_samples(0)) == bytes(g[tag + "_port0_sha256"]).decode()
+
+
+@pytest.mark.parametrize("kw", [{"cw": 1, "nt": 600}, {"magnetic": True, "dl": True, "gain": True}],
+                         ids=["cw", "magnetic"])
+def test_mixed_variants_f64_bit_exact(tmp_path, kw):
+    """Continuous-wave TF/SF over its ramp and into the steady sine (600
+    steps), and magnetic media beside Drude-Lorentz and gain cells, in the
+    fixed physics: engine = oracle bit for bit, graph-replayed blocks
+    included."""
+    nt = kw.pop("nt", 200)
+    case = cases.mixed(str(tmp_path), flags=FIXED, nt=nt, **kw)
+    print("mixed", sorted(kw), nt, "steps", flush=True)
+    e, o = run_both(case, nt)
+    assert_fields_equal(e, o)
+    for q in range(2):
+        assert np.array_equal(e.port_samples(q), o.port_samples(q))
+
+
+def test_sources_sharing_a_cell_f64_bit_exact(tmp_path):
+    """Three soft sources on one (cell, component), one on another component
+    of the same cell and one elsewhere: added one after another in source
+    order, as the reference does, so the engine equals the oracle bit for
+    bit (no lost contribution, no other rounding)."""
+    from cpsfdtd_amd.case import CaseWriter
+    d = 10e-9
+    w = CaseWriter(str(tmp_path))
+    w.materials([(1, 1, 0, 0), (2.25, 1, 0, 0)])
+    w.grid((24, 20, 18), (d, d, d), [])
+    w.cpml(4, m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.source(400e-9, 3e-15, 6e-15)
+    for amp, ph, comp, cell in ((1.0, 0.0, 2, (12, 10, 9)), (-0.37, 0.4, 2, (12, 10, 9)), (0.9, 1.1, 1, (12, 10, 9)),
+                                (2.5, 2.0, 2, (12, 10, 9)), (0.5, 0.0, 0, (7, 6, 5))):
+        w.point_source(*cell, comp=comp, amp=amp, phase=ph)
+    case = w.run(300, flags=FIXED)
+    e, o = run_both(case, 300)
+    assert_fields_equal(e, o, "shared source cell")
+    assert np.abs(o.field(2)).max() > 0

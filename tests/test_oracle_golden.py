@@ -136,3 +136,50 @@ def test_port_text_matches_reference_exports(oracle_lib, tmp_path):
             assert got[k] == want[k], (q, k)
     assert len(set(bytes(g["port0_ID_file"]).decode().split())) > 1  # the line crosses two media
     assert "\n" in port_text.fixture(g, 1)["Ex_file"].rstrip("\n")  # the 2-D plane prints rows of Nw
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def test_cw_tables_match_reference(oracle_lib, tmp_path):
+    """The continuous-wave TF/SF form (TF_SF.cpp:917-926): 5 periods of
+    Gaussian ramp, then a plain sine; Einc/Hinc over both branches bit for
+    bit (600 steps; the ramp ends at step 437)."""
+    g = _load("ref_extra.npz")
+    o = oracle_lib.OracleSim(cases.mixed(str(tmp_path), cw=1, nt=600))
+    e, h = o.tfsf_wave()
+    n = len(g["cw_tfsf_einc"])
+    assert n == 601
+    assert np.array_equal(e[:n], g["cw_tfsf_einc"]) and np.array_equal(h[:n], g["cw_tfsf_hinc"])
+    assert np.array_equal(o.tfsf_vec(), g["cw_tfsf_vec"][:12])
+    # both branches are present: unit-amplitude sine after the ramp
+    assert 0.99 < np.abs(e[440:600]).max() <= 1.0 and np.abs(e[:40]).max() < 1e-2
+
+
+@pytest.mark.parametrize("tag,kw", [("cw", {"cw": 1}), ("mag", {"magnetic": True})])
+def test_tiny_variants_run_bit_exact(oracle_lib, tmp_path, tag, kw):
+    """The reference's own loop on tests.cases.tiny with continuous-wave
+    TF/SF (300 steps) and with magnetic media (mu_r 2 and 1.5, sigma_M 1e9
+    ohm/m; 100 steps): fields and port samples bit for bit (SHA-256)."""
+    g = _load("ref_extra.npz")
+    nt = int(g[tag + "_nt"][0])
+    o = oracle_lib.OracleSim(cases.tiny(str(tmp_path), nt=nt, **kw))
+    o.step(nt)
+    for c in range(6):
+        f = o.field(c)
+        assert _sha(f.ravel()) == bytes(g[tag + "_field_sha256"][c]).decode(), (c, np.linalg.norm(f),
+                                                                                g[tag + "_field_l2"][c])
+    assert _sha(o.port_samples(0)) == bytes(g[tag + "_port0_sha256"]).decode()
+
+
+def test_magnetic_coefficients_match_reference(oracle_lib, tmp_path):
+    """aH/bH with mu_r != 1 and a magnetic conductivity (Structure.cpp:352-357)
+    bit for bit; the case does exercise them (aH < 1 where sigma_M > 0, bH
+    scaled by 1/mu_r)."""
+    g = _load("ref_extra.npz")
+    o = oracle_lib.OracleSim(cases.mixed(str(tmp_path), magnetic=True))
+    aE, bE, aH, bH = o.coefs()
+    assert np.array_equal(np.concatenate([aE, bE, aH, bH]), g["mag_coef"])
+    assert aH[0] == 1.0 and 0.98 < aH[3] < 0.999
+    assert abs(bH[1] * 2.0 / bH[0] - 1) < 1e-12
