@@ -72,6 +72,7 @@ SIGNATURES = {
     "cfd_port_points": ([_vp, C.c_int, _ip], C.c_int),
     "cfd_port_samples": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_port_gather": ([_vp, C.c_int, _dp], C.c_int),
+    "cfd_port_gather_static": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_port_dft": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_port_intensity": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_gain_read": ([_vp, _dp], C.c_int),
@@ -306,6 +307,13 @@ class Engine:
         n = _check(library().cfd_port_npts(self.h, q))
         a = np.zeros((n, 6), np.float64)
         _check(library().cfd_port_gather(self.h, q, a.ctypes.data_as(_dp)))
+        return a
+
+    def port_gather_static(self, q):
+        """port_gather through cfd_port_gather_static (no allocation)."""
+        n = _check(library().cfd_port_npts(self.h, q))
+        a = np.zeros((n, 6), np.float64)
+        _check(library().cfd_port_gather_static(self.h, q, a.ctypes.data_as(_dp)))
         return a
 
     def port_dft(self, q):

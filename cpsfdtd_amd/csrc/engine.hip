@@ -177,6 +177,10 @@ public:
     virtual void read_region(int comp, const int *box, double *host) = 0;
     virtual void port_samples(int q, double *host) = 0;
     virtual void port_gather(int q, double *host) = 0;
+    // the same values staged through caller-given device memory (out: npts * 6
+    // doubles); allocates nothing
+    virtual void port_gather_via(int q, double *out, double *host) = 0;
+    virtual int port_npts(int q) const = 0;
     virtual void port_dft(int q, double *host) = 0;
     virtual void port_intensity(int q, double *host) = 0;
     virtual void gain_read(double *host) = 0;
@@ -852,6 +856,15 @@ public:
         HIP_OK(hipStreamSynchronize(stream));
         for (size_t t = 0; t < tmp.size(); t++) host[t] = (double)tmp[t];
     }
+    void port_gather_via(int q, double *out, double *host) override {
+        const PortDesc<R> &p = dports.at(q);
+        if (!p.npts) return;
+        hipLaunchKernelGGL((k_gather<R, double>), dim3((p.npts + 255) / 256), dim3(256), 0, stream, d, p.cell, p.npts, out);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(host, out, (size_t)p.npts * 6 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    int port_npts(int q) const override { return dports.at(q).npts; }
     void port_dft(int q, double *host) override {
         const PortDesc<R> &p = dports.at(q);
         if (!p.dft) throw Error("no DFT frequencies configured");
@@ -1071,6 +1084,30 @@ public:
 }  // namespace cfd
 
 // ============================================================================
+// Staging memory of cfd_port_gather_static: a file-scope device array, so that
+// entry point allocates nothing.  Each GPU the library runs on holds its own
+// copy (kStaticGatherDoubles doubles, 8 MiB) until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (on the host it is
+// a stand-in, not a device address): it uses the address hipGetSymbolAddress
+// returns, looked up once per device with that device current.
+namespace cfd {
+constexpr size_t kStaticGatherDoubles = size_t(1) << 20;
+}
+static __device__ __align__(256) double scratch[cfd::kStaticGatherDoubles];
+
+namespace cfd {
+constexpr int kMaxDevices = 64;
+struct StaticScratch {
+    std::mutex mu;  // calls on one device share its array: one at a time
+    double *ptr[kMaxDevices] = {};
+};
+inline StaticScratch &static_scratch() {
+    static StaticScratch s;
+    return s;
+}
+}  // namespace cfd
+
+// ============================================================================
 // C ABI
 // ============================================================================
 using cfd::SimBase;
@@ -1277,6 +1314,28 @@ int cfd_port_gather(cfd_sim *s, int port, double *host)
     NEED(s);
     if (port < 0 || port >= (int)s->impl->m.ports.size() || !host) { cfd::g_error = "no such port"; return -1; }
     return guard([&] { s->impl->port_gather(port, host); });
+}
+int cfd_port_gather_static(cfd_sim *s, int port, double *host)
+{
+    NEED(s);
+    if (port < 0 || port >= (int)s->impl->m.ports.size() || !host) { cfd::g_error = "no such port"; return -1; }
+    return guard([&] {
+        const size_t need = (size_t)s->impl->port_npts(port) * 6;
+        if (need > cfd::kStaticGatherDoubles)
+            throw cfd::Error("port has " + std::to_string(need / 6) + " points; the static gather holds at most " +
+                             std::to_string(cfd::kStaticGatherDoubles / 6) + " (use cfd_port_gather)");
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= cfd::kMaxDevices) throw cfd::Error("device index out of range for the static gather");
+        cfd::StaticScratch &st = cfd::static_scratch();
+        std::lock_guard<std::mutex> lk(st.mu);
+        if (!st.ptr[dev]) {  // first call on this device: its copy of the array
+            void *p = nullptr;
+            HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            st.ptr[dev] = static_cast<double *>(p);
+        }
+        s->impl->port_gather_via(port, st.ptr[dev], host);
+    });
 }
 int cfd_port_dft(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_dft(port, host); }); }
 int cfd_port_intensity(cfd_sim *s, int port, double *host) { NEED(s); return guard([&] { s->impl->port_intensity(port, host); }); }

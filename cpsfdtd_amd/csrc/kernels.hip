@@ -874,14 +874,14 @@ __global__ __launch_bounds__(256) void k_tfsf(Dev<R> d0, const Dev<R> *devs, con
 }
 
 // the six field values at a list of cells, [point][6] (Ports.cpp:88-99 gather
-// for a caller that reads the current values)
-template <class R>
-__global__ __launch_bounds__(256) void k_gather(Dev<R> d, const long long *cell, int npts, R *out)
+// for a caller that reads the current values), written as O (R, or double)
+template <class R, class O = R>
+__global__ __launch_bounds__(256) void k_gather(Dev<R> d, const long long *cell, int npts, O *out)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= npts) return;
     const long long p = cell[t];
-    for (int c = 0; c < 6; c++) out[(long long)t * 6 + c] = d.F[c][p];
+    for (int c = 0; c < 6; c++) out[(long long)t * 6 + c] = (O)d.F[c][p];
 }
 
 // Non-finite values in the six fields (the serving loop's watchdog): one

@@ -114,6 +114,15 @@ int cfd_port_samples(cfd_sim *s, int port, double *host);       /* [steps][npts]
 int cfd_port_gather(cfd_sim *s, int port, double *host);        /* [npts][6] current values:
                                                                    *Port::Update_Fields_on_Port Ports.cpp:88-99,342-353
                                                                    read back (port_point[ii].Ex ... .Hz) */
+/* cfd_port_gather without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same values and layout as
+ * cfd_port_gather; the values are staged through a fixed device array of
+ * 1 Mi doubles per GPU (held until the process ends), so a port may have at
+ * most 174762 points (error otherwise).  Calls on one GPU share that array:
+ * the library serialises them within a process; calls from two processes on
+ * one GPU would race.  Call with the simulation's device current. */
+int cfd_port_gather_static(cfd_sim *s, int port, double *host);
 int cfd_port_dft(cfd_sim *s, int port, double *host);           /* [npts][n_freq][6][2] */
 int cfd_port_intensity(cfd_sim *s, int port, double *host);     /* [steps] sum of |E|^2 */
 /* Spectral energy through a plane port, J/Hz, one value per DFT frequency:
