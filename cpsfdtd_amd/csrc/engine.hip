@@ -107,10 +107,10 @@ inline bool stream_fields(size_t field_bytes) {
 // Merged half-step launch (k_full_h / k_full_e): cs = interior stretch as
 // scalars, nt = streamed fields; B = batched instances (blockIdx.y).
 template <class R, bool B>
-void launch_full(hipStream_t s, bool h, bool cs, bool nt, dim3 grid, const Dev<R> &d0, const Dev<R> *devs,
-                 const TileGrid &g, const Box &L, const LeanStretch<R> &st)
+void launch_full(hipStream_t s, bool h, bool cs, bool nt, unsigned ninst, const Dev<R> &d0, const Dev<R> *devs,
+                 const MGrid &g, const Box &L, const LeanStretch<R> &st)
 {
-    const dim3 blk(TX, TY);
+    const dim3 grid((unsigned)g.per_plane, (unsigned)g.planes, ninst), blk(256);
 #define CFD_FULL(K, CS, NT) hipLaunchKernelGGL((K<R, CS, NT, B>), grid, blk, 0, s, d0, devs, g, L, st)
     if (h) {
         if (cs) { if (nt) CFD_FULL(k_full_h, true, true); else CFD_FULL(k_full_h, true, false); }
@@ -481,6 +481,30 @@ public:
         g.gz = m.Nz;
         return g;
     }
+    // Tiles of a merged half-step (see MGrid): 32 x 8 tiles in the x-edge
+    // columns that hold the x CPML slabs, 64 x 4 tiles between them; one
+    // plane's tile origins, x fastest within each of the two kinds.
+    // CFD_TILE_EDGE32=0 gives 64-wide tiles everywhere (A/B runs).
+    std::vector<unsigned> mgrid_tiles() const {
+        const char *v = std::getenv("CFD_TILE_EDGE32");
+        const bool edge32 = !v || std::string(v) != "0";
+        const int n0 = edge32 ? m.npml[0][0] : 0, n1 = edge32 ? m.npml[0][1] : 0;
+        const int xa = n0 ? (n0 + 1 + 31) / 32 * 32 : 0;            // past the low slab (E cells up to n0)
+        const int wide = std::max(0, (n1 ? m.Nx - n1 : m.Nx) - xa);  // up to the high slab
+        const int nw = n1 ? wide / 64 : (wide + 63) / 64;
+        const int xb = xa + 64 * nw;
+        if (m.Nx > 0x7fff || m.Ny > 0xffff) throw Error("grid too large for the merged tile table");
+        std::vector<unsigned> t;
+        for (int y0 = 0; y0 < m.Ny; y0 += 4)
+            for (int b = 0; b < nw; b++) t.push_back((unsigned)(xa + 64 * b) | (unsigned)y0 << 15);
+        for (int y0 = 0; y0 < m.Ny; y0 += 8) {
+            for (int x0 = 0; x0 < xa; x0 += 32) t.push_back((unsigned)x0 | (unsigned)y0 << 15 | 1u << 31);
+            for (int x0 = xb; x0 < m.Nx; x0 += 32) t.push_back((unsigned)x0 | (unsigned)y0 << 15 | 1u << 31);
+        }
+        return t;
+    }
+    MGrid mgH{}, mgE{};
+
     // Fused-step update forms (CFD_UPDATE_KERNEL selects one, for A/B checks):
     //   split             lean interior box, CPML shell boxes, special cells
     //                     (default in FP64)
@@ -577,6 +601,12 @@ public:
         box_set(shellE, shellE_set, shellE_blocks);
         leanH_cs = lean_constant(leanH, false, leanH_st);
         leanE_cs = lean_constant(leanE, true, leanE_st);
+        {
+            const std::vector<unsigned> t = mgrid_tiles();
+            const unsigned *td = upload<unsigned>(t);
+            mgH = MGrid{td, (int)t.size(), 0, m.Nz};
+            mgE = MGrid{td, (int)t.size(), 1, std::max(0, m.Nz - 1)};
+        }
         // every Drude-Lorentz / gain cell, in device order; cells outside the
         // interior box carry the CPML flag
         std::vector<long long> sp;
@@ -610,13 +640,14 @@ public:
         }
         const Box &L = h ? leanH : leanE;
         if (form == kFormMerged) {
-            const TileGrid g = tiles();
-            const unsigned nb = (unsigned)(g.gx * g.gy * g.gz);
-            timer.run(stream, h ? "full_h" : "full_e", [&] {
-                launch_full<R, false>(stream, h, h ? leanH_cs : leanE_cs, nt_fields, dim3(nb), d, nullptr, g, L,
-                                      h ? leanH_st : leanE_st);
-            });
-            HIP_OK(hipGetLastError());
+            const MGrid &g = h ? mgH : mgE;
+            if (g.per_plane && g.planes) {
+                timer.run(stream, h ? "full_h" : "full_e", [&] {
+                    launch_full<R, false>(stream, h, h ? leanH_cs : leanE_cs, nt_fields, 1, d, nullptr, g, L,
+                                          h ? leanH_st : leanE_st);
+                });
+                HIP_OK(hipGetLastError());
+            }
             if (!h && nspecial) {
                 timer.run(stream, "special_e", [&] {
                     launch_special<R, false>(stream, dim3((nspecial + 255) / 256), d, nullptr, special, nspecial, nullptr,
@@ -1012,20 +1043,20 @@ public:
     void step(int count) override {
         const Sim<R> &a = *sims[0];
         const unsigned ni = (unsigned)sims.size();
-        const TileGrid g = a.tiles();
-        const unsigned nb = (unsigned)(g.gx * g.gy * g.gz);
         const Dev<R> d0 = a.d;  // by-value argument, unused by the batched kernels
+        auto update = [&](bool h) {
+            const MGrid &mg = h ? a.mgH : a.mgE;
+            if (mg.per_plane && mg.planes)
+                launch_full<R, true>(stream, h, h ? a.leanH_cs : a.leanE_cs, nt_fields, ni, d0, devs, mg,
+                                     h ? a.leanH : a.leanE, h ? a.leanH_st : a.leanE_st);
+        };
         for (int c = 0; c < count; c++) {
             const int n = done;
             if (n >= a.m.nt) throw Error("run finished: all " + std::to_string(a.m.nt) + " steps done");
-            timer.run(stream, "full_h", [&] {
-                launch_full<R, true>(stream, true, a.leanH_cs, nt_fields, dim3(nb, ni), d0, devs, g, a.leanH, a.leanH_st);
-            });
+            timer.run(stream, "full_h", [&] { update(true); });
             HIP_OK(hipGetLastError());
             tfsf(true, n);
-            timer.run(stream, "full_e", [&] {
-                launch_full<R, true>(stream, false, a.leanE_cs, nt_fields, dim3(nb, ni), d0, devs, g, a.leanE, a.leanE_st);
-            });
+            timer.run(stream, "full_e", [&] { update(false); });
             HIP_OK(hipGetLastError());
             if (max_special) {
                 timer.run(stream, "special_e", [&] {

@@ -717,30 +717,79 @@ __device__ __forceinline__ const Dev<R> &instance(const Dev<R> &d0, const Dev<R>
 // the stencil with CPML terms, testing only the axes whose slabs the tile
 // reaches (workgroup-uniform mask).  Drude-Lorentz / gain cells are left to
 // the special pass (they store their own value back).
+//
+// Tiles.  Rows are cut into 64-wide tiles (64 x 4 cells, a wavefront per
+// x-row) except in the two x-edge columns holding the x CPML slabs, which are
+// cut into 32 x 8 tiles: a tile on the CPML path then carries 20 cells
+// outside the slab beside the 12 in it, instead of 52 (with 12-cell slabs
+// half of all 64-wide tiles would take the CPML path).  The edge columns are
+// multiples of 32 cells, so every row segment still starts on a 128-byte
+// line in FP32.
+//
+// Decode.  The launch is 2-D: blockIdx.x is the tile within its plane, read
+// from a per-plane table of tile origins (one scalar load per workgroup, no
+// integer division), blockIdx.y the plane, blockIdx.z a batch instance.
+//
+// Measured on the 256^3 FP32 workload (MI355X): 32-wide edge tiles took
+// k_full_e 157 -> 134 us and k_full_h 146 -> 133 us (same decode); numbering
+// the tiles so that each XCD takes a contiguous z-slab (which helps a
+// uniform grid) made both 12-22 us slower here.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool tile_inside(const Box &L, int x0, int y0, int k)
+struct MGrid {
+    const unsigned *tiles;  // per plane: x0 | y0 << 15 | (32-wide ? 1 << 31 : 0)
+    int per_plane;          // tiles per plane
+    int k_lo, planes;       // planes [k_lo, k_lo + planes): H from 0, E from 1
+};
+
+struct MTile {
+    int x0, y0, k;   // tile origin and plane
+    int tw, th;      // 64 x 4 or 32 x 8
+    int i, j;        // this thread's cell
+};
+template <bool B>
+__device__ __forceinline__ void mtile_of(const MGrid &g, MTile &t)
 {
-    return x0 >= L.i0 && x0 + TX <= L.i1 && y0 >= L.j0 && y0 + TY <= L.j1 && k >= L.k0 && k < L.k1;
+    const unsigned e = g.tiles[blockIdx.x];
+    const bool narrow = e >> 31;
+    t.x0 = (int)(e & 0x7fffu);
+    t.y0 = (int)((e >> 15) & 0xffffu);
+    t.k = g.k_lo + (int)blockIdx.y;
+    t.tw = narrow ? 32 : 64;
+    t.th = narrow ? 8 : 4;
+    const int lin = threadIdx.x;
+    t.i = t.x0 + (narrow ? lin & 31 : lin & 63);
+    t.j = t.y0 + (narrow ? lin >> 5 : lin >> 6);
+}
+// the batch instance of a merged launch (blockIdx.z)
+template <class R, bool B>
+__device__ __forceinline__ const Dev<R> &instance_z(const Dev<R> &d0, const Dev<R> *devs)
+{
+    if constexpr (B) return devs[blockIdx.z];
+    else return d0;
+}
+
+__device__ __forceinline__ bool tile_inside(const Box &L, const MTile &t)
+{
+    return t.x0 >= L.i0 && t.x0 + t.tw <= L.i1 && t.y0 >= L.j0 && t.y0 + t.th <= L.j1 && t.k >= L.k0 && t.k < L.k1;
 }
 template <class R>
-__device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, int x0, int y0, int k, bool efield)
+__device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, const MTile &t, bool efield)
 {
     const int o = efield ? 1 : 0;  // E slabs start at index 1 on the low side
     unsigned m = 0;
-    if ((d.npml[0][0] && x0 < d.npml[0][0] + o) || (d.npml[0][1] && x0 + TX > d.Nx - d.npml[0][1])) m |= 1u;
-    if ((d.npml[1][0] && y0 < d.npml[1][0] + o) || (d.npml[1][1] && y0 + TY > d.Ny - d.npml[1][1])) m |= 2u;
-    if ((d.npml[2][0] && k < d.npml[2][0] + o) || (d.npml[2][1] && k >= d.Nz - d.npml[2][1])) m |= 4u;
+    if ((d.npml[0][0] && t.x0 < d.npml[0][0] + o) || (d.npml[0][1] && t.x0 + t.tw > d.Nx - d.npml[0][1])) m |= 1u;
+    if ((d.npml[1][0] && t.y0 < d.npml[1][0] + o) || (d.npml[1][1] && t.y0 + t.th > d.Ny - d.npml[1][1])) m |= 2u;
+    if ((d.npml[2][0] && t.k < d.npml[2][0] + o) || (d.npml[2][1] && t.k >= d.Nz - d.npml[2][1])) m |= 4u;
     return m;
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs, TileGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
-    const Dev<R> &d = instance<R, B>(d0, devs);
-    int bx, by, k;
-    if (!tile_of(g, bx, by, k)) return;
-    const int x0 = bx * TX, y0 = by * TY;
-    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+    const Dev<R> &d = instance_z<R, B>(d0, devs);
+    MTile t;
+    mtile_of<B>(g, t);
+    const int i = t.i, j = t.j, k = t.k;
     if (i >= d.Nx || j >= d.Ny) return;
     const long long p = cell_index(d.px, d.py, i, j, k);
     const long long pi1 = (d.pbx && i + 1 == d.Nx) ? p + 1 - d.Nx : p + 1;
@@ -753,14 +802,14 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
     const R ex_j1 = d.F[0][pj1], ez_j1 = d.F[2][pj1], ex_k1 = d.F[0][pk1];
     const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
     R hx = ld_once<NT>(&d.F[3][p]), hy = ld_once<NT>(&d.F[4][p]), hz = ld_once<NT>(&d.F[5][p]);
-    if (tile_inside(L, x0, y0, k)) {
+    if (tile_inside(L, t)) {
         const R aH = d.aH[id], bH = d.bH[id];
         hx = aH * hx - bH * (ez_j1 - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (ey_k1 - ey) * lean_st<CS>(d, c, S_cHxy, k);
         hy = aH * hy - bH * (ex_k1 - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (ez_i1 - ez) * lean_st<CS>(d, c, S_cHyz, i);
         hz = aH * hz + bH * (ex_j1 - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (ey_i1 - ey) * lean_st<CS>(d, c, S_cHzy, i);
     } else {
-        h_cell(d, true, d.has_pml ? tile_pml_mask(d, x0, y0, k, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
-               ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz, id);
+        h_cell(d, true, d.has_pml ? tile_pml_mask(d, t, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1, ex_k1,
+               ey_k1, ey_i1, ez_i1, hx, hy, hz, id);
     }
     st_once<NT>(&d.F[3][p], hx);
     st_once<NT>(&d.F[4][p], hy);
@@ -768,13 +817,12 @@ __global__ __launch_bounds__(TX *TY) void k_full_h(Dev<R> d0, const Dev<R> *devs
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs, TileGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
-    const Dev<R> &d = instance<R, B>(d0, devs);
-    int bx, by, k;
-    if (!tile_of(g, bx, by, k)) return;
-    const int x0 = bx * TX, y0 = by * TY;
-    const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+    const Dev<R> &d = instance_z<R, B>(d0, devs);
+    MTile t;
+    mtile_of<B>(g, t);
+    const int i = t.i, j = t.j, k = t.k;
     if (i < d.ie0 || i >= d.Nx || j < d.je0 || j >= d.Ny || k < 1 || k >= d.Nz) return;
     const long long p = cell_index(d.px, d.py, i, j, k);
     const long long pim = (i == 0) ? p + d.Nx - 1 : p - 1;
@@ -789,7 +837,7 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
     // each branch after its loads (hoisted ahead of the branch, it made the
     // CPML tiles wait for the ID byte before issuing their psi loads)
     const R E0[3] = {E[0], E[1], E[2]};
-    if (tile_inside(L, x0, y0, k)) {
+    if (tile_inside(L, t)) {
         const R aE = d.aE[id], bE = d.bE[id];
         E[0] = aE * E[0] + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
         E[1] = aE * E[1] + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
@@ -797,8 +845,8 @@ __global__ __launch_bounds__(TX *TY) void k_full_e(Dev<R> d0, const Dev<R> *devs
         const bool keep = special_id(d, id) | (id == 2);  // ID 2 without the ADE module is left as is
         for (int q = 0; q < 3; q++) E[q] = keep ? E0[q] : id == 4 ? (R)0 : E[q];
     } else {
-        e_cell(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, x0, y0, k, true) : 0u, i, j, k, p, id, hx, hy, hz,
-               hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E, true);
+        e_cell(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, t, true) : 0u, i, j, k, p, id, hx, hy, hz, hx_jm,
+               hz_jm, hx_km, hy_km, hy_im, hz_im, E, true);
         const bool sp = special_id(d, id);
         for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
     }
