@@ -116,6 +116,16 @@ SIGNATURES = {
     "cfd_image_write_png": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_last_error": ([], C.c_char_p),
     "cfd_config_check": ([C.POINTER(_Config)], C.c_int),
+    "cfd_pack_create": ([C.c_int, C.POINTER(_vp)], C.c_int),
+    "cfd_pack_destroy": ([_vp], C.c_int),
+    "cfd_pack_add": ([_vp, _vp], C.c_int),
+    "cfd_pack_remove": ([_vp, _vp], C.c_int),
+    "cfd_pack_size": ([_vp], C.c_int),
+    "cfd_pack_step": ([_vp, C.c_int], C.c_int),
+    "cfd_pack_sync": ([_vp], C.c_int),
+    "cfd_pack_query": ([_vp], C.c_int),
+    "cfd_pack_set_timing": ([_vp, C.c_int], C.c_int),
+    "cfd_pack_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
 }
 
 _lib = None
@@ -355,6 +365,55 @@ def _kernel_times(fn, h):
     cnt = np.zeros(cap, np.int32)
     n = _check(fn(h, C.cast(names, C.c_void_p), ms.ctypes.data_as(_dp), cnt.ctypes.data_as(_ip), cap))
     return {bytes(names[i]).split(b"\0")[0].decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+
+class Pack:
+    """Simulations of any grid size and physics stepped together, each pass of
+    a step one launch over all members (cfd_pack_*).  Members are Engines of
+    the pack's precision; add() at step 0 (or a multiple of 16 DFT strides),
+    remove() before stepping or freeing a member alone."""
+
+    def __init__(self, precision=CFD_F64):
+        h = C.c_void_p()
+        _check(library().cfd_pack_create(precision, C.byref(h)))
+        self.h = h
+        self.precision = precision
+        self.members = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().cfd_pack_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return _check(library().cfd_pack_size(self.h))
+
+    def add(self, engine):
+        _check(library().cfd_pack_add(self.h, engine.h))
+        self.members.append(engine)
+
+    def remove(self, engine):
+        _check(library().cfd_pack_remove(self.h, engine.h))
+        self.members.remove(engine)
+
+    def step(self, n=1):
+        _check(library().cfd_pack_step(self.h, n))
+
+    def sync(self):
+        _check(library().cfd_pack_sync(self.h))
+
+    def set_timing(self, on):
+        _check(library().cfd_pack_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        return _kernel_times(library().cfd_pack_kernel_times, self.h)
 
 
 class Batch:

@@ -155,6 +155,7 @@ public:
     Model m;
     int precision = 64;
     bool member = false;     // instance of a batch: stepped by the batch only
+    bool packed = false;     // member of a pack: stepped by the pack only
     bool own_stream = true;
     uint64_t tf_sig = 14695981039346656037ull;    // TF/SF target lists, terms and waveforms
     uint64_t port_sig = 14695981039346656037ull;  // port cells, frequencies, recording
@@ -1112,6 +1113,254 @@ public:
     }
 };
 
+// ----------------------------------------------------------------------------
+// A pack: simulations of any grid size and physics (one precision) stepped
+// together, every pass of a step one launch over all members (the packed
+// kernels in kernels.hip).  Members keep their own device state and stay
+// owned by their handles; the pack holds the tables that describe them to the
+// packed kernels, rebuilt when a member joins or leaves, and replays
+// sequences of kPackBlock steps from a captured HIP graph while the
+// membership is unchanged (a member's time index comes from device memory).
+// No reference counterpart: the reference runs one simulation per process
+// (main.cpp:115-343).
+class PackBase {
+public:
+    virtual ~PackBase() = default;
+    hipStream_t stream = nullptr;
+    KernelTimer timer;
+    virtual void add(SimBase *s) = 0;
+    virtual void remove(SimBase *s) = 0;
+    virtual void step(int count) = 0;
+    virtual int size() const = 0;
+};
+
+constexpr int kPackBlock = 128;  // steps per captured sequence
+
+template <class R>
+class Pack : public PackBase {
+public:
+    std::vector<Sim<R> *> mem;
+    std::vector<void *> allocs;  // tables of the current membership
+    PackMember<R> *dmem = nullptr;
+    int *nstep = nullptr;
+    unsigned long long *workH = nullptr, *workE = nullptr;
+    int nworkH = 0, nworkE = 0;
+    int *st_tfH = nullptr, *st_tfE = nullptr, *st_sp = nullptr, *st_src = nullptr, *st_port = nullptr;
+    int nb_tfH = 0, nb_tfE = 0, nb_sp = 0, nb_src = 0, nb_port = 0;
+    bool any_dft = false, nt_fields = false, dirty = true;
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+
+    Pack() {
+        HIP_OK(hipStreamCreate(&stream));
+        if (const char *v = std::getenv("CFD_GRAPHS")) use_graphs = std::string(v) != "0";
+    }
+    ~Pack() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (Sim<R> *s : mem) s->packed = false;
+        drop_tables();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int size() const override { return (int)mem.size(); }
+
+    void drop_tables() {
+        std::lock_guard<std::mutex> lk(capture_mutex());
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        for (void *p : allocs) (void)hipFreeAsync(p, stream);
+        allocs.clear();
+        if (stream) (void)hipStreamSynchronize(stream);
+    }
+    template <class T>
+    T *upload(const std::vector<T> &v) {
+        void *p = nullptr;
+        const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+        {
+            std::lock_guard<std::mutex> lk(capture_mutex());
+            HIP_OK(hipMallocAsync(&p, bytes, stream));
+        }
+        allocs.push_back(p);
+        if (!v.empty()) HIP_OK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        return static_cast<T *>(p);
+    }
+
+    void add(SimBase *b) override {
+        Sim<R> *s = dynamic_cast<Sim<R> *>(b);
+        if (!s) throw Error("pack: the simulation's precision differs from the pack's");
+        if (s->member) throw Error("pack: a batch member is stepped by its batch");
+        if (s->packed) throw Error("pack: the simulation is already in a pack");
+        if (s->done >= s->m.nt) throw Error("pack: the simulation's run has finished");
+        if (!s->m.freqs.empty()) {
+            // the packed port kernels stage DFT samples in slots counted from
+            // step 0 in groups of 16: join on such a boundary, nothing staged
+            s->dft_flush();
+            if (s->done % (16 * s->m.dft_stride)) throw Error("pack: join at step 0 or a multiple of 16 DFT strides");
+        }
+        HIP_OK(hipStreamSynchronize(s->stream));  // set-up or earlier steps done before the pack uses it
+        HIP_OK(hipStreamSynchronize(stream));
+        mem.push_back(s);
+        s->packed = true;
+        dirty = true;
+    }
+    void remove(SimBase *b) override {
+        auto it = std::find(mem.begin(), mem.end(), b);
+        if (it == mem.end()) throw Error("pack: not a member");
+        HIP_OK(hipStreamSynchronize(stream));  // its last steps done before its handle is used again
+        (*it)->packed = false;
+        mem.erase(it);
+        dirty = true;
+    }
+
+    void rebuild() {
+        HIP_OK(hipStreamSynchronize(stream));
+        drop_tables();
+        std::vector<PackMember<R>> pm;
+        std::vector<int> nst;
+        std::vector<unsigned long long> wh, we;
+        std::vector<int> tfH{0}, tfE{0}, sp{0}, src{0}, port{0};
+        size_t field_bytes = 0;
+        any_dft = false;
+        for (size_t q = 0; q < mem.size(); q++) {
+            Sim<R> &S = *mem[q];
+            PackMember<R> P{};
+            P.d = S.d;
+            P.mh = S.mgH; P.me = S.mgE;
+            P.Lh = S.leanH; P.Le = S.leanE;
+            P.sh = S.leanH_st; P.se = S.leanE_st;
+            P.csH = S.leanH_cs; P.csE = S.leanE_cs;
+            P.nt = S.m.nt;
+            P.ntfE = S.ntfE; P.ntfH = S.ntfH; P.tablen = S.tablen;
+            P.tfE = S.tfE; P.tfH = S.tfH;
+            P.tcoef = S.tcoef; P.tdelay = S.tdelay; P.tprof = S.tprof; P.einc = S.einc; P.hinc = S.hinc;
+            P.special = S.special; P.nspecial = S.nspecial;
+            P.nsgroup = S.nsgroup; P.sgroup = S.sgroup; P.sidx = S.sidx; P.swave = S.swave;
+            P.ports = S.port_tab; P.nports = (int)S.dports.size();
+            P.dft_stride = S.m.dft_stride; P.nf = (int)S.m.freqs.size();
+            P.twE = S.twE; P.twH = S.twH;
+            pm.push_back(P);
+            nst.push_back(S.done);
+            any_dft = any_dft || P.nf > 0;
+            const unsigned long long sq = (unsigned long long)q << 48;
+            for (int k = 0; k < S.mgH.planes; k++)
+                for (int t = 0; t < S.mgH.per_plane; t++)
+                    wh.push_back(sq | (unsigned long long)(S.mgH.k_lo + k) << kPackPlaneShift | (unsigned long long)t);
+            for (int k = 0; k < S.mgE.planes; k++)
+                for (int t = 0; t < S.mgE.per_plane; t++)
+                    we.push_back(sq | (unsigned long long)(S.mgE.k_lo + k) << kPackPlaneShift | (unsigned long long)t);
+            tfH.push_back(tfH.back() + (S.ntfH + 255) / 256);
+            tfE.push_back(tfE.back() + (S.ntfE + 255) / 256);
+            sp.push_back(sp.back() + (S.nspecial + 255) / 256);
+            src.push_back(src.back() + (S.nsgroup + 63) / 64);
+            port.push_back(port.back() + S.port_blocks);
+            field_bytes += 6 * (size_t)S.d.slab * (S.m.Nz + 1) * sizeof(R);
+        }
+        if (wh.size() > 0x7fffffffu || we.size() > 0x7fffffffu) throw Error("pack: too many workgroups");
+        nt_fields = stream_fields(field_bytes);
+        dmem = upload(pm);
+        nstep = upload(nst);
+        workH = upload(wh); nworkH = (int)wh.size();
+        workE = upload(we); nworkE = (int)we.size();
+        st_tfH = upload(tfH); nb_tfH = tfH.back();
+        st_tfE = upload(tfE); nb_tfE = tfE.back();
+        st_sp = upload(sp); nb_sp = sp.back();
+        st_src = upload(src); nb_src = src.back();
+        st_port = upload(port); nb_port = port.back();
+        HIP_OK(hipStreamSynchronize(stream));
+        dirty = false;
+    }
+
+    void launch_step(int q) {
+        const int nm = (int)mem.size();
+        const dim3 blk(256);
+        if (nworkH) timer.run(stream, "pack_h", [&] {
+            if (nt_fields) hipLaunchKernelGGL((k_pack_h<R, true>), dim3(nworkH), blk, 0, stream, dmem, workH, nstep, q);
+            else hipLaunchKernelGGL((k_pack_h<R, false>), dim3(nworkH), blk, 0, stream, dmem, workH, nstep, q);
+        });
+        if (nb_tfH) timer.run(stream, "pack_tfsf_h", [&] {
+            hipLaunchKernelGGL(k_pack_tfsf<R>, dim3(nb_tfH), blk, 0, stream, dmem, st_tfH, nm, nstep, q, 1);
+        });
+        if (nworkE) timer.run(stream, "pack_e", [&] {
+            if (nt_fields) hipLaunchKernelGGL((k_pack_e<R, true>), dim3(nworkE), blk, 0, stream, dmem, workE, nstep, q);
+            else hipLaunchKernelGGL((k_pack_e<R, false>), dim3(nworkE), blk, 0, stream, dmem, workE, nstep, q);
+        });
+        if (nb_sp) timer.run(stream, "pack_special_e", [&] {
+            hipLaunchKernelGGL(k_pack_special<R>, dim3(nb_sp), blk, 0, stream, dmem, st_sp, nm, nstep, q);
+        });
+        if (nb_tfE) timer.run(stream, "pack_tfsf_e", [&] {
+            hipLaunchKernelGGL(k_pack_tfsf<R>, dim3(nb_tfE), blk, 0, stream, dmem, st_tfE, nm, nstep, q, 0);
+        });
+        if (nb_src) timer.run(stream, "pack_sources", [&] {
+            hipLaunchKernelGGL(k_pack_sources<R>, dim3(nb_src), dim3(64), 0, stream, dmem, st_src, nm, nstep, q);
+        });
+        if (nb_port) {
+            timer.run(stream, "pack_ports", [&] {
+                hipLaunchKernelGGL(k_pack_port<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
+            });
+            if (any_dft) timer.run(stream, "pack_dft", [&] {
+                hipLaunchKernelGGL(k_pack_dft<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
+            });
+        }
+        HIP_OK(hipGetLastError());
+    }
+    void launch_sequence(int count) {
+        for (int q = 0; q < count; q++) launch_step(q);
+        const int nm = (int)mem.size();
+        hipLaunchKernelGGL(k_pack_advance, dim3((nm + 63) / 64), dim3(64), 0, stream, nstep, nm, count);
+        HIP_OK(hipGetLastError());
+    }
+    void capture() {
+        hipGraph_t g = nullptr;
+        std::lock_guard<std::mutex> lk(capture_mutex());
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        try {
+            launch_sequence(kPackBlock);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_OK(hipStreamEndCapture(stream, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_OK(e);
+    }
+
+    // count steps of every member (asynchronous); a member whose run ends
+    // within them idles for the rest
+    void step(int count) override {
+        if (count <= 0 || mem.empty()) return;
+        if (dirty) rebuild();
+        int left = count;
+        while (left > 0) {
+            if (use_graphs && !timer.on && left >= kPackBlock) {
+                if (!gexec) capture();
+                HIP_OK(hipGraphLaunch(gexec, stream));
+                left -= kPackBlock;
+            } else {
+                const int c = std::min(left, kPackBlock);
+                launch_sequence(c);
+                left -= c;
+            }
+        }
+        for (Sim<R> *s : mem) {
+            s->done = std::min(s->done + count, s->m.nt);
+            s->dft_pending = 0;
+            if (s->m.freqs.empty() || s->done == 0) continue;
+            // staged DFT samples not yet added in: the sampled steps after the
+            // last flush (flushes come at every 16th sampled step and at the
+            // run's last sampled step)
+            const int st = s->m.dft_stride;
+            const int last_sampled = (s->done - 1) / st * st;
+            const int slot = (last_sampled / st) % 16;
+            const int run_last = (s->m.nt - 1) / st * st;
+            if (slot != 15 && last_sampled != run_last) {
+                s->dft_pending = slot + 1;
+                s->dft_first = last_sampled - slot * st;
+            }
+        }
+    }
+};
+
 }  // namespace cfd
 
 // ============================================================================
@@ -1153,6 +1402,11 @@ struct cfd_batch {
     std::vector<std::unique_ptr<cfd_sim>> members;  // borrowed views of the instances
 };
 
+struct cfd_pack {
+    std::unique_ptr<cfd::PackBase> impl;
+    int precision = 0;
+};
+
 template <class F>
 static int guard(F &&f)
 {
@@ -1181,6 +1435,10 @@ static int guard(F &&f)
         NEED(s);                                                                      \
         if ((s)->impl->member) {                                                      \
             cfd::g_error = "batch member: the batch steps its instances (cfd_batch_step)"; \
+            return -1;                                                                \
+        }                                                                             \
+        if ((s)->impl->packed) {                                                      \
+            cfd::g_error = "pack member: the pack steps it (cfd_pack_step)";          \
             return -1;                                                                \
         }                                                                             \
     } while (0)
@@ -1239,6 +1497,7 @@ int cfd_destroy(cfd_sim *s)
 {
     if (!s) return 0;
     if (!s->owned) { cfd::g_error = "a batch member is released with its batch (cfd_batch_destroy)"; return -1; }
+    if (s->impl->packed) { cfd::g_error = "a pack member: remove it from its pack first (cfd_pack_remove)"; return -1; }
     delete s;
     return 0;
 }
@@ -1493,6 +1752,86 @@ int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *lau
         HIP_OK(hipStreamSynchronize(b->impl->stream));
         b->impl->timer.collect();
         for (auto &kv : b->impl->timer.acc) {
+            if (n >= cap) break;
+            snprintf(names[n], 32, "%s", kv.first.c_str());
+            ms[n] = kv.second.first;
+            launches[n] = kv.second.second;
+            n++;
+        }
+    });
+    return rc ? rc : n;
+}
+
+// ---------------------------------------------------------------------------
+// packs
+int cfd_pack_create(int precision, cfd_pack **out)
+{
+    if (!out) { cfd::g_error = "null argument"; return -1; }
+    *out = nullptr;
+    return guard([&] {
+        if (precision != CFD_F32 && precision != CFD_F64) throw cfd::Error("precision must be 32 or 64");
+        auto p = std::make_unique<cfd_pack>();
+        if (precision == CFD_F32) p->impl = std::make_unique<cfd::Pack<float>>();
+        else p->impl = std::make_unique<cfd::Pack<double>>();
+        p->precision = precision;
+        *out = p.release();
+    });
+}
+int cfd_pack_destroy(cfd_pack *p)
+{
+    if (!p) return 0;
+    delete p;
+    return 0;
+}
+#define NEED_PACK(p)                                    \
+    do {                                                \
+        if (!(p) || !(p)->impl) {                       \
+            cfd::g_error = "null pack handle";          \
+            return -1;                                  \
+        }                                               \
+    } while (0)
+int cfd_pack_add(cfd_pack *p, cfd_sim *s)
+{
+    NEED_PACK(p);
+    NEED(s);
+    return guard([&] { p->impl->add(s->impl); });
+}
+int cfd_pack_remove(cfd_pack *p, cfd_sim *s)
+{
+    NEED_PACK(p);
+    NEED(s);
+    return guard([&] { p->impl->remove(s->impl); });
+}
+int cfd_pack_size(cfd_pack *p) { NEED_PACK(p); return p->impl->size(); }
+int cfd_pack_step(cfd_pack *p, int count) { NEED_PACK(p); return guard([&] { p->impl->step(count); }); }
+int cfd_pack_sync(cfd_pack *p) { NEED_PACK(p); return guard([&] { HIP_OK(hipStreamSynchronize(p->impl->stream)); }); }
+int cfd_pack_query(cfd_pack *p)
+{
+    NEED_PACK(p);
+    const hipError_t e = hipStreamQuery(p->impl->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    cfd::g_error = std::string("HIP: ") + hipGetErrorString(e);
+    return -1;
+}
+int cfd_pack_set_timing(cfd_pack *p, int on)
+{
+    NEED_PACK(p);
+    return guard([&] {
+        HIP_OK(hipStreamSynchronize(p->impl->stream));
+        p->impl->timer.collect();
+        p->impl->timer.acc.clear();
+        p->impl->timer.on = on != 0;
+    });
+}
+int cfd_pack_kernel_times(cfd_pack *p, char (*names)[32], double *ms, int *launches, int cap)
+{
+    NEED_PACK(p);
+    int n = 0;
+    int rc = guard([&] {
+        HIP_OK(hipStreamSynchronize(p->impl->stream));
+        p->impl->timer.collect();
+        for (auto &kv : p->impl->timer.acc) {
             if (n >= cap) break;
             snprintf(names[n], 32, "%s", kv.first.c_str());
             ms[n] = kv.second.first;

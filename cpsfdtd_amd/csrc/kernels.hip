@@ -746,14 +746,14 @@ struct MTile {
     int tw, th;      // 64 x 4 or 32 x 8
     int i, j;        // this thread's cell
 };
-template <bool B>
-__device__ __forceinline__ void mtile_of(const MGrid &g, MTile &t)
+// tile `tile` of the plane table at plane k (absolute)
+__device__ __forceinline__ void mtile_at(const MGrid &g, unsigned tile, int k, MTile &t)
 {
-    const unsigned e = g.tiles[blockIdx.x];
+    const unsigned e = g.tiles[tile];
     const bool narrow = e >> 31;
     t.x0 = (int)(e & 0x7fffu);
     t.y0 = (int)((e >> 15) & 0xffffu);
-    t.k = g.k_lo + (int)blockIdx.y;
+    t.k = k;
     t.tw = narrow ? 32 : 64;
     t.th = narrow ? 8 : 4;
     const int lin = threadIdx.x;
@@ -783,12 +783,10 @@ __device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, const MTile &
     return m;
 }
 
-template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(256) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+// one merged H tile-plane
+template <class R, bool CS, bool NT>
+__device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Box &L, const LeanStretch<R> &c)
 {
-    const Dev<R> &d = instance_z<R, B>(d0, devs);
-    MTile t;
-    mtile_of<B>(g, t);
     const int i = t.i, j = t.j, k = t.k;
     if (i >= d.Nx || j >= d.Ny) return;
     const long long p = cell_index(d.px, d.py, i, j, k);
@@ -817,11 +815,18 @@ __global__ __launch_bounds__(256) void k_full_h(Dev<R> d0, const Dev<R> *devs, M
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(256) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance_z<R, B>(d0, devs);
     MTile t;
-    mtile_of<B>(g, t);
+    mtile_at(g, blockIdx.x, g.k_lo + (int)blockIdx.y, t);
+    h_tile<R, CS, NT>(d, t, L, c);
+}
+
+// one merged E tile-plane
+template <class R, bool CS, bool NT>
+__device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Box &L, const LeanStretch<R> &c)
+{
     const int i = t.i, j = t.j, k = t.k;
     if (i < d.ie0 || i >= d.Nx || j < d.je0 || j >= d.Ny || k < 1 || k >= d.Nz) return;
     const long long p = cell_index(d.px, d.py, i, j, k);
@@ -853,6 +858,15 @@ __global__ __launch_bounds__(256) void k_full_e(Dev<R> d0, const Dev<R> *devs, M
     st_once<NT>(&d.F[0][p], E[0]);
     st_once<NT>(&d.F[1][p], E[1]);
     st_once<NT>(&d.F[2][p], E[2]);
+}
+
+template <class R, bool CS, bool NT, bool B = false>
+__global__ __launch_bounds__(256) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+{
+    const Dev<R> &d = instance_z<R, B>(d0, devs);
+    MTile t;
+    mtile_at(g, blockIdx.x, g.k_lo + (int)blockIdx.y, t);
+    e_tile<R, CS, NT>(d, t, L, c);
 }
 
 // Drude-Lorentz and gain cells, all of them: device cell index, with bit
@@ -902,6 +916,18 @@ struct TfTarget {
 // TF/SF face corrections (TF_SF.cpp:988-1089 for E, :1201-1301 for H): one
 // thread per corrected field value, its face terms applied in the reference's
 // face order.  Incident values are interpolated in FP64.
+template <class R>
+__device__ __forceinline__ void tfsf_apply(const Dev<R> &d, const TfTarget &g, const double *coef, const double *delay,
+                                           const double *prof, const double *tab, int tablen, int n)
+{
+    R *F = d.F[g.comp];
+    R v = F[g.cell];
+    for (int q = g.first; q < g.first + g.count; q++) {
+        const double inc = inc_interp(tab, tablen, (double)n - delay[q]);
+        v = v + (R)(coef[q] * inc * prof[q]);
+    }
+    F[g.cell] = v;
+}
 template <class R, bool B = false>
 __global__ __launch_bounds__(256) void k_tfsf(Dev<R> d0, const Dev<R> *devs, const TfTarget *tg, int ntg,
                                               const double *coef, const double *delay, const double *prof,
@@ -911,14 +937,7 @@ __global__ __launch_bounds__(256) void k_tfsf(Dev<R> d0, const Dev<R> *devs, con
     if (nbase) n += *nbase;  // replayed step blocks: step = block start + offset
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= ntg) return;
-    const TfTarget g = tg[t];
-    R *F = d.F[g.comp];
-    R v = F[g.cell];
-    for (int q = g.first; q < g.first + g.count; q++) {
-        const double inc = inc_interp(tab, tablen, (double)n - delay[q]);
-        v = v + (R)(coef[q] * inc * prof[q]);
-    }
-    F[g.cell] = v;
+    tfsf_apply(d, tg[t], coef, delay, prof, tab, tablen, n);
 }
 
 // the six field values at a list of cells, [point][6] (Ports.cpp:88-99 gather
@@ -967,17 +986,22 @@ struct SourceGroup {
     int comp, first, count, pad;  // members idx[first .. first + count)
 };
 template <class R>
+__device__ __forceinline__ void source_apply(const Dev<R> &d, const SourceGroup &G, const int *idx, const double *wave,
+                                             int stride, int n)
+{
+    R *F = d.F[G.comp];
+    R v = F[G.cell];
+    for (int q = G.first; q < G.first + G.count; q++) v = v + (R)wave[(long long)idx[q] * stride + n];
+    F[G.cell] = v;
+}
+template <class R>
 __global__ void k_sources(Dev<R> d, const SourceGroup *grp, int ngroups, const int *idx, const double *wave,
                           int stride, const int *nbase, int n)
 {
     if (nbase) n += *nbase;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
-    const SourceGroup G = grp[g];
-    R *F = d.F[G.comp];
-    R v = F[G.cell];
-    for (int q = G.first; q < G.first + G.count; q++) v = v + (R)wave[(long long)idx[q] * stride + n];
-    F[G.cell] = v;
+    source_apply(d, grp[g], idx, wave, stride, n);
 }
 
 // One port of a simulation as the port kernels see it.  Points of all ports
@@ -995,18 +1019,17 @@ struct PortDesc {
 // Port gather (Ports.cpp:88-99, 342-353) into the time-series record and the
 // DFT batch, and the per-step sum of |E|^2 (Ports.cpp:512-522) reduced across
 // the workgroup with wavefront shuffles into one partial per workgroup.
-template <class R, bool B = false>
-__global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, const PortDesc<R> *ports, int nports,
-                                              const int *nbase, int n, int slot)
+// one workgroup (blk) of the port sampling of one simulation's ports; every
+// thread of the workgroup calls it (it synchronises the workgroup)
+template <class R>
+__device__ __forceinline__ void port_block(const Dev<R> &d, const PortDesc<R> *ports, int nports, int blk0, int n,
+                                           int slot)
 {
-    if (nbase) n += *nbase;
-    const Dev<R> &d = instance<R, B>(d0, devs);
-    if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     __shared__ double red[4];
     int q = 0;
-    while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
+    while (q + 1 < nports && blk0 >= ports[q + 1].bstart) q++;
     const PortDesc<R> P = ports[q];
-    const int blk = blockIdx.x - P.bstart;
+    const int blk = blk0 - P.bstart;
     const int t = blk * 256 + threadIdx.x;
     double isq = 0.0;
     if (t < P.npts) {
@@ -1025,21 +1048,28 @@ __global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, con
     __syncthreads();
     if (threadIdx.x == 0) P.ipart[(long long)n * P.nblk + blk] = ((red[0] + red[1]) + red[2]) + red[3];
 }
+template <class R, bool B = false>
+__global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, const PortDesc<R> *ports, int nports,
+                                              const int *nbase, int n, int slot)
+{
+    if (nbase) n += *nbase;
+    const Dev<R> &d = instance<R, B>(d0, devs);
+    if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
+    port_block(d, ports, nports, (int)blockIdx.x, n, slot);
+}
 
 // Running DFT over a batch of nb sampled steps n0, n0+stride, ...: each FP64
 // accumulator is read once, gets the batch's terms in step order (the same
 // additions, in the same order, as accumulating every sampled step), and is
 // written once.  E samples carry phase 2 pi f (n+1) dt, H samples (n+1/2) dt.
-template <class R, bool B = false>
-__global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nports, int nb, const int *nbase, int n0,
-                                             int stride, const double *twE, const double *twH, int nf, int ntot)
+template <class R>
+__device__ __forceinline__ void dft_block(const PortDesc<R> *ports, int nports, int blk0, int nb, int n0, int stride,
+                                          const double *twE, const double *twH, int nf, int ntot)
 {
-    if (nbase) n0 += *nbase;
-    if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     int q = 0;
-    while (q + 1 < nports && (int)blockIdx.x >= ports[q + 1].bstart) q++;
+    while (q + 1 < nports && blk0 >= ports[q + 1].bstart) q++;
     const PortDesc<R> P = ports[q];
-    const int t = (blockIdx.x - P.bstart) * 256 + threadIdx.x;
+    const int t = (blk0 - P.bstart) * 256 + threadIdx.x;
     if (t >= P.npts) return;
     const long long np_ = P.npts;
     for (int f = 0; f < nf; f++) {
@@ -1059,6 +1089,177 @@ __global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nport
         }
         for (int w = 0; w < 12; w++) acc[w * np_] = a[w];
     }
+}
+template <class R, bool B = false>
+__global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nports, int nb, const int *nbase, int n0,
+                                             int stride, const double *twE, const double *twH, int nf, int ntot)
+{
+    if (nbase) n0 += *nbase;
+    if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
+    dft_block(ports, nports, (int)blockIdx.x, nb, n0, stride, twE, twH, nf, ntot);
+}
+
+// ---------------------------------------------------------------------------
+// Packed launches: one launch per pass of the step over every member of a
+// pack, members of any grid size and physics (the serving loop's requests).
+// Each member keeps its own device state; the update kernels find their
+// member, plane and tile in a per-workgroup work table, the smaller passes
+// their member in per-member workgroup ranges.  A member's time index is its
+// step at the start of the launch sequence (nstep[member], device memory)
+// plus the offset q of the step within the sequence, so one captured sequence
+// serves every quantum; a member whose run has ended skips its workgroups.
+// Every cell, target and port point is updated by the arithmetic of the
+// single-simulation kernels, so a member's results do not depend on the other
+// members (bit for bit).
+// ---------------------------------------------------------------------------
+template <class R>
+struct PackMember {
+    Dev<R> d;
+    MGrid mh, me;                   // merged update tiles
+    Box Lh, Le;                     // interior boxes
+    LeanStretch<R> sh, se;          // interior stretch factors (when csH / csE)
+    int csH, csE;
+    int nt;                         // run length
+    int ntfE, ntfH, tablen;         // TF/SF targets per half-step, waveform length
+    const TfTarget *tfE, *tfH;
+    const double *tcoef, *tdelay, *tprof, *einc, *hinc;
+    const long long *special;       // Drude-Lorentz / gain cells (kSpecialPml flag)
+    int nspecial;
+    int nsgroup;                    // point-source groups
+    const SourceGroup *sgroup;
+    const int *sidx;
+    const double *swave;
+    const PortDesc<R> *ports;       // this member's port table
+    int nports, dft_stride, nf;     // ports; DFT stride and frequencies (0: no DFT)
+    const double *twE, *twH;
+};
+
+// the member owning workgroup b: largest s with start[s] <= b (start has n + 1
+// ascending entries, start[n] = total workgroups)
+__device__ __forceinline__ int pack_member(const int *start, int n, int b)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// work entry of the update kernels: member << 48 | plane << 24 | tile
+constexpr int kPackPlaneShift = 24;
+
+template <class R, bool NT>
+__global__ __launch_bounds__(256) void k_pack_h(const PackMember<R> *pm, const unsigned long long *work,
+                                                const int *nstep, int q)
+{
+    const unsigned long long w = work[blockIdx.x];
+    const int s = (int)(w >> 48);
+    const PackMember<R> &P = pm[s];
+    if (nstep[s] + q >= P.nt) return;
+    MTile t;
+    mtile_at(P.mh, (unsigned)(w & 0xffffffu), (int)((w >> kPackPlaneShift) & 0xffffffu), t);
+    if (P.csH) h_tile<R, true, NT>(P.d, t, P.Lh, P.sh);
+    else h_tile<R, false, NT>(P.d, t, P.Lh, P.sh);
+}
+
+template <class R, bool NT>
+__global__ __launch_bounds__(256) void k_pack_e(const PackMember<R> *pm, const unsigned long long *work,
+                                                const int *nstep, int q)
+{
+    const unsigned long long w = work[blockIdx.x];
+    const int s = (int)(w >> 48);
+    const PackMember<R> &P = pm[s];
+    if (nstep[s] + q >= P.nt) return;
+    MTile t;
+    mtile_at(P.me, (unsigned)(w & 0xffffffu), (int)((w >> kPackPlaneShift) & 0xffffffu), t);
+    if (P.csE) e_tile<R, true, NT>(P.d, t, P.Le, P.se);
+    else e_tile<R, false, NT>(P.d, t, P.Le, P.se);
+}
+
+// TF/SF corrections of every member (h: the H half-step's)
+template <class R>
+__global__ __launch_bounds__(256) void k_pack_tfsf(const PackMember<R> *pm, const int *start, int nmem,
+                                                   const int *nstep, int q, int h)
+{
+    const int s = pack_member(start, nmem, blockIdx.x);
+    const PackMember<R> &P = pm[s];
+    const int n = nstep[s] + q;
+    if (n >= P.nt) return;
+    const int t = (blockIdx.x - start[s]) * 256 + threadIdx.x;
+    if (t >= (h ? P.ntfH : P.ntfE)) return;
+    tfsf_apply(P.d, (h ? P.tfH : P.tfE)[t], P.tcoef, P.tdelay, P.tprof, h ? P.einc : P.hinc, P.tablen, n);
+}
+
+// Drude-Lorentz and gain cells of every member (the generic cell update: the
+// members' pole counts may differ)
+template <class R>
+__global__ __launch_bounds__(256) void k_pack_special(const PackMember<R> *pm, const int *start, int nmem,
+                                                      const int *nstep, int q)
+{
+    const int s = pack_member(start, nmem, blockIdx.x);
+    const PackMember<R> &P = pm[s];
+    if (nstep[s] + q >= P.nt) return;
+    const int t = (blockIdx.x - start[s]) * 256 + threadIdx.x;
+    if (t >= P.nspecial) return;
+    const Dev<R> &d = P.d;
+    const long long v = P.special[t];
+    const long long p = v & ((1LL << kSpecialPml) - 1);
+    const int k = (int)(p / d.slab);
+    const long long r = p - (long long)k * d.slab;
+    const int j = (int)(r / d.px), i = (int)(r - (long long)j * d.px);
+    e_full(d, i, j, k, (v >> kSpecialPml) & 1 ? 7u : 0u);
+}
+
+template <class R>
+__global__ __launch_bounds__(64) void k_pack_sources(const PackMember<R> *pm, const int *start, int nmem,
+                                                     const int *nstep, int q)
+{
+    const int s = pack_member(start, nmem, blockIdx.x);
+    const PackMember<R> &P = pm[s];
+    const int n = nstep[s] + q;
+    if (n >= P.nt) return;
+    const int g = (blockIdx.x - start[s]) * 64 + threadIdx.x;
+    if (g >= P.nsgroup) return;
+    source_apply(P.d, P.sgroup[g], P.sidx, P.swave, P.nt + 1, n);
+}
+
+// port sampling of every member: the DFT staging slot is the member's
+// sampled-step count modulo 16 (its flushes come every 16 sampled steps)
+template <class R>
+__global__ __launch_bounds__(256) void k_pack_port(const PackMember<R> *pm, const int *start, int nmem,
+                                                   const int *nstep, int q)
+{
+    const int s = pack_member(start, nmem, blockIdx.x);
+    const PackMember<R> &P = pm[s];
+    const int n = nstep[s] + q;
+    if (n >= P.nt) return;
+    const int slot = (P.nf && n % P.dft_stride == 0) ? (n / P.dft_stride) % 16 : -1;
+    port_block(P.d, P.ports, P.nports, (int)blockIdx.x - start[s], n, slot);
+}
+
+// DFT flush of every member whose staging is full this step, or whose last
+// sampled step of the run this is
+template <class R>
+__global__ __launch_bounds__(256) void k_pack_dft(const PackMember<R> *pm, const int *start, int nmem,
+                                                  const int *nstep, int q)
+{
+    const int s = pack_member(start, nmem, blockIdx.x);
+    const PackMember<R> &P = pm[s];
+    const int n = nstep[s] + q;
+    if (n >= P.nt || !P.nf || n % P.dft_stride) return;
+    const int slot = (n / P.dft_stride) % 16;
+    const int last = (P.nt - 1) / P.dft_stride * P.dft_stride;
+    if (slot != 15 && n != last) return;
+    dft_block(P.ports, P.nports, (int)blockIdx.x - start[s], slot + 1, n - slot * P.dft_stride, P.dft_stride, P.twE,
+              P.twH, P.nf, P.nt);
+}
+
+// every member's step base moves on by the length of the launched sequence
+__global__ void k_pack_advance(int *nstep, int nmem, int count)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nmem) nstep[s] += count;
 }
 
 }  // namespace cfd

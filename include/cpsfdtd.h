@@ -156,6 +156,26 @@ int cfd_batch_device_bytes(cfd_batch *b, size_t *bytes);
 int cfd_batch_set_timing(cfd_batch *b, int on);
 int cfd_batch_kernel_times(cfd_batch *b, char (*names)[32], double *ms, int *launches, int cap);
 
+/* Packs: simulations of any grid size and physics (one precision) stepped
+ * together, each pass of a step one launch over all members (the serving
+ * loop's form).  No reference counterpart: the reference runs one simulation
+ * per process (main.cpp:115-343).  A member keeps its handle and device
+ * state (read it with the calls above once removed or the pack synchronised);
+ * while in a pack it is stepped only by cfd_pack_step.  A simulation with a
+ * running DFT joins at step 0 or at a multiple of 16 DFT strides.  Members
+ * give the same results, bit for bit, as when stepped alone. */
+typedef struct cfd_pack cfd_pack;
+int cfd_pack_create(int precision, cfd_pack **out);
+int cfd_pack_destroy(cfd_pack *p);                  /* members stay valid handles */
+int cfd_pack_add(cfd_pack *p, cfd_sim *s);
+int cfd_pack_remove(cfd_pack *p, cfd_sim *s);
+int cfd_pack_size(cfd_pack *p);
+int cfd_pack_step(cfd_pack *p, int count);          /* asynchronous; a finished member idles */
+int cfd_pack_sync(cfd_pack *p);
+int cfd_pack_query(cfd_pack *p);                    /* 1 done, 0 running */
+int cfd_pack_set_timing(cfd_pack *p, int on);
+int cfd_pack_kernel_times(cfd_pack *p, char (*names)[32], double *ms, int *launches, int cap);
+
 /* Serving: requests (one cfd_config each) admitted first-come-first-served
  * while their device memory fits hbm_budget, then stepped in quanta of
  * `quantum` steps on their own streams so that requests of different shapes
