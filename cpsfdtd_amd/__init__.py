@@ -529,7 +529,8 @@ def dft_flux(dft, axis, scale):
 
 
 class _ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("quantum", C.c_int), ("policy", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("quantum", C.c_int), ("policy", C.c_int),
+                ("streams", C.c_int)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -540,8 +541,8 @@ class Server:
     """Serving loop (cfd_server_*): submit cases, then poll or wait; results
     are the port |E|^2 series and DFTs, on the host once a request is done."""
 
-    def __init__(self, hbm_budget=0, max_active=0, quantum=0, policy=0):
-        o = _ServerOpts(hbm_budget, max_active, quantum, policy)
+    def __init__(self, hbm_budget=0, max_active=0, quantum=0, policy=0, streams=False):
+        o = _ServerOpts(hbm_budget, max_active, quantum, policy, int(streams))
         h = C.c_void_p()
         if library().cfd_server_create(C.byref(o), C.byref(h)) < 0:
             raise EngineError(library().cfd_server_last_error().decode())

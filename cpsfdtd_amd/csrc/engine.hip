@@ -1150,6 +1150,7 @@ public:
     bool any_dft = false, nt_fields = false, dirty = true;
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
+    int calls_since_rebuild = 0;  // step() calls with the current membership
 
     Pack() {
         HIP_OK(hipStreamCreate(&stream));
@@ -1329,10 +1330,17 @@ public:
     // within them idles for the rest
     void step(int count) override {
         if (count <= 0 || mem.empty()) return;
-        if (dirty) rebuild();
+        if (dirty) {
+            rebuild();
+            calls_since_rebuild = 0;
+        }
+        // a membership that changes at every call (a serving loop admitting
+        // and retiring requests) would pay a capture per call: capture from
+        // the second call with the same members on
+        const bool graphs = use_graphs && !timer.on && calls_since_rebuild++ > 0;
         int left = count;
         while (left > 0) {
-            if (use_graphs && !timer.on && left >= kPackBlock) {
+            if (graphs && left >= kPackBlock) {
                 if (!gexec) capture();
                 HIP_OK(hipGraphLaunch(gexec, stream));
                 left -= kPackBlock;

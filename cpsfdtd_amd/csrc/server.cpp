@@ -1,10 +1,11 @@
 // server.cpp -- serving loop over the C ABI: requests (one simulation each)
-// are admitted first-come-first-served while their device memory fits a
-// budget, then stepped in quanta on their own HIP streams, so the kernels of
-// different requests overlap on the GPU (continuous batching across shapes).
-// A finished request's port results are copied to host memory and its device
-// memory freed at once.  One scheduler thread; submit / poll / result are
-// safe from any thread.
+// are admitted (first come first served, or least work first) while their
+// device memory fits a budget, then stepped in quanta: every running request
+// of a precision is a member of one pack, so each pass of a step is one
+// launch over all of them whatever their sizes (continuous batching across
+// shapes; requests join and leave between quanta).  A finished request's
+// port results are copied to host memory and its device memory freed at
+// once.  submit / poll / result are safe from any thread.
 //
 // The reference runs one simulation per process (main.cpp:115-343); this has
 // no counterpart there.
@@ -81,7 +82,8 @@ struct Request {
     cfd_sim *sim = nullptr;
     size_t bytes = 0;
     int nt = 0, done = 0;
-    bool inflight = false;
+    bool inflight = false;      // per-stream mode: a quantum is running
+    bool in_pack = false;       // packed mode: a member of the precision's pack
     double t_submit = 0, t_start = 0, t_done = 0;
     std::string error;
     std::vector<std::vector<double>> intensity, dft;  // per port
@@ -258,6 +260,58 @@ struct cfd_server {
         }
     }
 
+    // stepping thread, packed mode: join newly admitted requests to their
+    // precision's pack, run one quantum of every pack (as many steps as the
+    // member with the fewest left needs, at most opts.quantum, so no member
+    // idles), then check and release finished or failed members
+    cfd_pack *packs[2] = {nullptr, nullptr};  // FP32, FP64
+    void pack_quantum(const std::vector<Request *> &act) {
+        for (Request *r : act) {
+            if (r->in_pack) continue;
+            cfd_pack *&p = packs[r->cfg->cfg.precision == CFD_F64 ? 1 : 0];
+            if ((!p && cfd_pack_create(r->cfg->cfg.precision, &p) < 0) || cfd_pack_add(p, r->sim) < 0) {
+                retire(r, cfd_last_error());
+                continue;
+            }
+            r->in_pack = true;
+        }
+        std::vector<Request *> mem;
+        int q = opts.quantum;
+        for (Request *r : act)
+            if (r->in_pack) {
+                mem.push_back(r);
+                q = std::min(q, r->nt - r->done);
+            }
+        if (mem.empty()) {
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+            return;
+        }
+        std::string err[2];
+        for (int w = 0; w < 2; w++)
+            if (packs[w] && cfd_pack_size(packs[w]) > 0 && cfd_pack_step(packs[w], q) < 0) err[w] = cfd_last_error();
+        for (int w = 0; w < 2; w++)
+            if (packs[w] && cfd_pack_size(packs[w]) > 0 && err[w].empty() && cfd_pack_sync(packs[w]) < 0)
+                err[w] = cfd_last_error();
+        for (Request *r : mem) {
+            const int w = r->cfg->cfg.precision == CFD_F64 ? 1 : 0;
+            r->done += q;
+            steps_launched += q;
+            std::string e = err[w];
+            if (e.empty()) {
+                // watchdog: a request whose fields went non-finite fails now,
+                // alone, instead of running on and returning garbage
+                const int bad = cfd_check_finite(r->sim);
+                if (bad < 0) e = cfd_last_error();
+                else if (bad > 0) e = "non-finite field values (" + std::to_string(bad) + ") by step " + std::to_string(r->done);
+            }
+            if (!e.empty() || r->done >= r->nt) {
+                cfd_pack_remove(packs[w], r->sim);
+                r->in_pack = false;
+                retire(r, e);
+            }
+        }
+    }
+
     void step_loop() {
         for (;;) {
             std::vector<Request *> act;
@@ -266,6 +320,10 @@ struct cfd_server {
                 cv_step.wait(lk, [&] { return stop || !active.empty(); });
                 if (stop) break;
                 act = active;
+            }
+            if (!opts.streams) {
+                pack_quantum(act);
+                continue;
             }
             // every idle request gets its next quantum (asynchronous, own stream);
             // a request retired here belongs to the host thread from then on
@@ -305,6 +363,11 @@ struct cfd_server {
                 if (r->done >= r->nt) retire(r, std::string());
             }
             if (!any) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+        // stopping: the packs let go of their members (shutdown fails them)
+        for (cfd_pack *&p : packs) {
+            if (p) cfd_pack_destroy(p);
+            p = nullptr;
         }
     }
     // stepping thread: hand a request over to the host thread
@@ -348,9 +411,10 @@ int cfd_server_create(const cfd_server_opts *o, cfd_server **out)
     // requests gave p50 21 s against 32 s with 64 and the same throughput (the
     // GPU is saturated either way; fewer at once finish early arrivals sooner);
     // 256-step quanta are two replayed 128-step graph blocks
-    s->opts.max_active = o && o->max_active > 0 ? o->max_active : 4;
+    s->opts.max_active = o && o->max_active > 0 ? o->max_active : 8;
     s->opts.quantum = o && o->quantum > 0 ? o->quantum : 256;
     s->opts.policy = o && o->policy == CFD_POLICY_SHORTEST ? CFD_POLICY_SHORTEST : CFD_POLICY_FIFO;
+    s->opts.streams = o && o->streams ? 1 : 0;
     cfd_server *p = s.get();
     p->stepper = std::thread([p] { p->step_loop(); });
     p->host = std::thread([p] { p->host_loop(); });

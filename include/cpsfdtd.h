@@ -177,9 +177,11 @@ int cfd_pack_set_timing(cfd_pack *p, int on);
 int cfd_pack_kernel_times(cfd_pack *p, char (*names)[32], double *ms, int *launches, int cap);
 
 /* Serving: requests (one cfd_config each) admitted first-come-first-served
- * while their device memory fits hbm_budget, then stepped in quanta of
- * `quantum` steps on their own streams so that requests of different shapes
- * run concurrently; a finished request's port results (|E|^2 series, DFT)
+ * while their device memory fits hbm_budget, then stepped in quanta of at
+ * most `quantum` steps, all running requests of one precision together as
+ * the members of a pack (one launch per pass over all of them; see
+ * cfd_pack_*), or, with `streams` set, each on its own stream; a finished
+ * request's port results (|E|^2 series, DFT)
  * are copied to host memory and its device memory freed.  A scheduler thread
  * runs inside the server; the calls below are thread-safe.  No reference
  * counterpart (the reference runs one simulation per process,
@@ -197,6 +199,7 @@ typedef struct {
     int max_active;      /* requests running at once (0: 4) */
     int quantum;         /* steps per scheduling slice (0: 256) */
     int policy;          /* CFD_POLICY_* (0: FIFO) */
+    int streams;         /* 1: each request on its own stream instead of packed launches */
 } cfd_server_opts;
 typedef struct cfd_server cfd_server;
 int cfd_server_create(const cfd_server_opts *opts, cfd_server **out);

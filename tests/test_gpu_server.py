@@ -1,8 +1,12 @@
 """The serving loop (cfd_server_*, cpsfdtd_amd.Server): requests of different
-shapes run concurrently on their own streams, in step quanta; each request's
-results must equal a solo run of the same case bit for bit (FP64 and FP32),
-a failing request must not disturb the others, and the memory budget must
-hold requests back."""
+shapes run together in step quanta, as members of one pack per precision
+(one launch per pass over all of them) or each on its own stream; each
+request's results must equal a solo run of the same case bit for bit (FP64
+and FP32), a failing request must not disturb the others, and the memory
+budget must hold requests back.  The first eight requests of the seeded c5
+stream run at their real sizes: FP64 against the oracle over their first 200
+steps (fixture), and served to the end against solo runs."""
+import hashlib
 import os
 
 import numpy as np
@@ -28,8 +32,9 @@ def make(root, k, params):
     return configs.c5(d, params)
 
 
-def test_server_results_equal_solo_runs(tmp_path):
-    srv = eng.Server(max_active=3, quantum=37)
+@pytest.mark.parametrize("streams", [False, True], ids=["packed", "streams"])
+def test_server_results_equal_solo_runs(tmp_path, streams):
+    srv = eng.Server(max_active=3, quantum=37, streams=streams)
     cases, ids = [], []
     for k, (p, prec) in enumerate(SMALL):
         case = make(tmp_path, k, p)
@@ -151,4 +156,50 @@ def test_server_rejects_a_malformed_request_at_submit(tmp_path):
     assert srv.stats()["queued"] == 0
     r = srv.submit(good, 64)
     assert srv.wait(r, 120.0) == eng.REQ_DONE
+    srv.close()
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def test_c5_first_requests_real_size(tmp_path):
+    """The first eight requests of the seeded c5 stream at their real grid
+    sizes (up to 239 x 313 x 210 cells, Drude-Lorentz, gain and dielectric
+    members): the FP64 engine equals the oracle's first 200 steps bit for bit
+    (fields, port DFT; |E|^2 sums within 1e-12), and served in FP32 to their
+    last step (up to 6954 steps) each result equals its solo run bit for bit."""
+    g = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c5_first8.npz"),
+                     allow_pickle=False))
+    steps = int(g["steps"][0])
+    rq = configs.c5_requests(8)
+    cases = []
+    for q, (_, p) in enumerate(rq):
+        case = make(tmp_path, q, p)
+        assert tuple(g["N%d" % q]) == p["N"] and case.nt == int(g["nt%d" % q][0])
+        cases.append(case)
+        print("c5 request %d %s %s: FP64 %d steps vs oracle" % (q, p["N"], p["kind"], steps), flush=True)
+        e = eng.Engine(case, 64)
+        e.step(steps)
+        for c in range(6):
+            assert _sha(e.field(c)) == bytes(g["field_sha256%d" % q][c]).decode(), (q, c)
+        ref = g["intensity%d" % q]
+        got = e.port_intensity(0)
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), q
+        d = e.port_dft(0)
+        assert _sha(np.stack([d.real, d.imag], axis=-1)) == bytes(g["dft_sha256%d" % q]).decode(), q
+        e.close()
+    print("c5: serving the eight requests in FP32", flush=True)
+    srv = eng.Server()
+    ids = [srv.submit(case, 32) for case in cases]
+    for rid in ids:
+        assert srv.wait(rid, 300.0) == eng.REQ_DONE, srv.error(rid)
+    for q, (rid, case) in enumerate(zip(ids, cases)):
+        e = eng.Engine(case, 32)
+        e.step(case.nt)
+        assert np.array_equal(srv.result(rid, 0, "intensity"), e.port_intensity(0)), q
+        assert np.array_equal(srv.result(rid, 0, "dft"), e.port_dft(0)), q
+        assert np.abs(e.port_intensity(0)).max() > 0
+        e.close()
+        srv.release(rid)
     srv.close()
