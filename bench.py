@@ -332,7 +332,7 @@ def batched_c4(steps=200, warmup=10, n=512):
             "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()}}
 
 
-def serving_c5(nreq=120, precision=32, quantum=256, policy=0):
+def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack_cells=0):
     """The first ``nreq`` requests of the serving workload's stream (Poisson
     20/s, log-uniform 32-320 edges, 500-8000 steps, sphere / Drude-Lorentz /
     gain mix), submitted at their arrival times to the server: latency from
@@ -348,7 +348,7 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0):
         os.makedirs(d)
         cases.append((t, configs.c5(d, p)))
     work = sum(p["N"][0] * p["N"][1] * p["N"][2] * p["steps"] for _, p in rq)
-    srv = eng.Server(quantum=quantum, policy=policy)
+    srv = eng.Server(quantum=quantum, policy=policy, max_active=max_active, pack_cells=pack_cells)
     try:
         t0 = time.perf_counter()
         ids = []
@@ -379,7 +379,9 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0):
             "arrival_span_s": round(rq[-1][0], 2), "makespan_s": round(span, 2),
             "demand_mcells": round(work / rq[-1][0] / 1e6, 1), "peak_active": stats["peak_active"],
             "peak_device_gb": round(stats["peak_bytes"] / 1e9, 2), "quantum": quantum,
-            "policy": "shortest work first" if policy == eng.POLICY_SHORTEST else "first come, first served"}
+            "policy": "shortest work first" if policy == eng.POLICY_SHORTEST else "first come, first served",
+            "stepping": "requests under %g cells packed (one launch per pass over all of them), larger ones on their "
+            "own streams" % pack_cells if pack_cells else "one stream per request"}
 
 
 def cpu_fifo_model(nreq, mcells_per_s):

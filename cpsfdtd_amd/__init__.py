@@ -124,6 +124,7 @@ SIGNATURES = {
     "cfd_pack_step": ([_vp, C.c_int], C.c_int),
     "cfd_pack_sync": ([_vp], C.c_int),
     "cfd_pack_query": ([_vp], C.c_int),
+    "cfd_pack_check_finite": ([_vp, C.POINTER(_vp), C.c_int, _ip], C.c_int),
     "cfd_pack_set_timing": ([_vp, C.c_int], C.c_int),
     "cfd_pack_kernel_times": ([_vp, C.c_void_p, _dp, _ip, C.c_int], C.c_int),
 }
@@ -415,6 +416,14 @@ class Pack:
     def kernel_times(self):
         return _kernel_times(library().cfd_pack_kernel_times, self.h)
 
+    def check_finite(self, engines=None):
+        """Non-finite field values of each given member (default: all)."""
+        engines = list(self.members if engines is None else engines)
+        sims = (_vp * len(engines))(*[e.h for e in engines])
+        counts = np.zeros(len(engines), np.int32)
+        _check(library().cfd_pack_check_finite(self.h, sims, len(engines), counts.ctypes.data_as(_ip)))
+        return counts.tolist()
+
 
 class Batch:
     """Simulations of one shape stepped together: each pass of a step is one
@@ -530,7 +539,7 @@ def dft_flux(dft, axis, scale):
 
 class _ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("quantum", C.c_int), ("policy", C.c_int),
-                ("streams", C.c_int)]
+                ("pack_cells", C.c_double)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -541,8 +550,8 @@ class Server:
     """Serving loop (cfd_server_*): submit cases, then poll or wait; results
     are the port |E|^2 series and DFTs, on the host once a request is done."""
 
-    def __init__(self, hbm_budget=0, max_active=0, quantum=0, policy=0, streams=False):
-        o = _ServerOpts(hbm_budget, max_active, quantum, policy, int(streams))
+    def __init__(self, hbm_budget=0, max_active=0, quantum=0, policy=0, pack_cells=0):
+        o = _ServerOpts(hbm_budget, max_active, quantum, policy, float(pack_cells))
         h = C.c_void_p()
         if library().cfd_server_create(C.byref(o), C.byref(h)) < 0:
             raise EngineError(library().cfd_server_last_error().decode())

@@ -1132,9 +1132,15 @@ public:
     virtual void remove(SimBase *s) = 0;
     virtual void step(int count) = 0;
     virtual int size() const = 0;
+    // non-finite field values of each given member (the serving loop's
+    // watchdog), counted on the pack's stream with one synchronisation
+    virtual void nonfinite(SimBase *const *sims, int n, int *counts) = 0;
 };
 
-constexpr int kPackBlock = 128;  // steps per captured sequence
+// Steps per captured sequence.  Short, so that a membership that changes at
+// every quantum (a serving loop) costs one short capture instead of
+// thousands of launches: a 256-step quantum is 16 replays.
+constexpr int kPackBlock = 16;
 
 template <class R>
 class Pack : public PackBase {
@@ -1150,7 +1156,6 @@ public:
     bool any_dft = false, nt_fields = false, dirty = true;
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
-    int calls_since_rebuild = 0;  // step() calls with the current membership
 
     Pack() {
         HIP_OK(hipStreamCreate(&stream));
@@ -1326,18 +1331,28 @@ public:
         HIP_OK(e);
     }
 
+    void nonfinite(SimBase *const *sims, int n, int *counts) override {
+        std::vector<Sim<R> *> ms;
+        for (int q = 0; q < n; q++) {
+            Sim<R> *s = dynamic_cast<Sim<R> *>(sims[q]);
+            if (!s || std::find(mem.begin(), mem.end(), s) == mem.end()) throw Error("pack: not a member");
+            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, stream, s->count_dev, 0);
+            hipLaunchKernelGGL(k_nonfinite<R>, dim3(1024), dim3(256), 0, stream, s->d, s->d.slab * (s->m.Nz + 1),
+                               s->count_dev);
+            ms.push_back(s);
+        }
+        HIP_OK(hipGetLastError());
+        for (int q = 0; q < n; q++)
+            HIP_OK(hipMemcpyAsync(counts + q, ms[q]->count_dev, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+
     // count steps of every member (asynchronous); a member whose run ends
     // within them idles for the rest
     void step(int count) override {
         if (count <= 0 || mem.empty()) return;
-        if (dirty) {
-            rebuild();
-            calls_since_rebuild = 0;
-        }
-        // a membership that changes at every call (a serving loop admitting
-        // and retiring requests) would pay a capture per call: capture from
-        // the second call with the same members on
-        const bool graphs = use_graphs && !timer.on && calls_since_rebuild++ > 0;
+        if (dirty) rebuild();
+        const bool graphs = use_graphs && !timer.on;
         int left = count;
         while (left > 0) {
             if (graphs && left >= kPackBlock) {
@@ -1811,6 +1826,19 @@ int cfd_pack_remove(cfd_pack *p, cfd_sim *s)
     return guard([&] { p->impl->remove(s->impl); });
 }
 int cfd_pack_size(cfd_pack *p) { NEED_PACK(p); return p->impl->size(); }
+int cfd_pack_check_finite(cfd_pack *p, cfd_sim *const *sims, int n, int *counts)
+{
+    NEED_PACK(p);
+    if (n < 0 || (n && (!sims || !counts))) { cfd::g_error = "null argument"; return -1; }
+    return guard([&] {
+        std::vector<SimBase *> v;
+        for (int q = 0; q < n; q++) {
+            if (!sims[q] || !sims[q]->impl) throw cfd::Error("null simulation handle");
+            v.push_back(sims[q]->impl);
+        }
+        p->impl->nonfinite(v.data(), n, counts);
+    });
+}
 int cfd_pack_step(cfd_pack *p, int count) { NEED_PACK(p); return guard([&] { p->impl->step(count); }); }
 int cfd_pack_sync(cfd_pack *p) { NEED_PACK(p); return guard([&] { HIP_OK(hipStreamSynchronize(p->impl->stream)); }); }
 int cfd_pack_query(cfd_pack *p)

@@ -1,9 +1,10 @@
 // server.cpp -- serving loop over the C ABI: requests (one simulation each)
 // are admitted (first come first served, or least work first) while their
-// device memory fits a budget, then stepped in quanta: every running request
-// of a precision is a member of one pack, so each pass of a step is one
-// launch over all of them whatever their sizes (continuous batching across
-// shapes; requests join and leave between quanta).  A finished request's
+// device memory fits a budget, then stepped in quanta, each request on its
+// own stream (the streams' small passes overlap on the GPU), or, for
+// requests below a size, as members of a pack, where each pass of a step is
+// one launch over all of them whatever their sizes (continuous batching
+// across shapes; requests join and leave between quanta).  A finished request's
 // port results are copied to host memory and its device memory freed at
 // once.  submit / poll / result are safe from any thread.
 //
@@ -79,11 +80,13 @@ struct Request {
     cfd_model *model = nullptr;  // built ahead of admission
     size_t estimate = 0;
     double work = 0;             // cells x steps, once the model is built
+    double cells = 0;            // Nx Ny Nz, once the model is built
     cfd_sim *sim = nullptr;
     size_t bytes = 0;
     int nt = 0, done = 0;
     bool inflight = false;      // per-stream mode: a quantum is running
-    bool in_pack = false;       // packed mode: a member of the precision's pack
+    bool in_pack = false;       // packed mode: a member of a pack
+    int pack = 0;               // which pack
     double t_submit = 0, t_start = 0, t_done = 0;
     std::string error;
     std::vector<std::vector<double>> intensity, dft;  // per port
@@ -176,7 +179,8 @@ struct cfd_server {
         int n[4] = {0, 0, 0, 0};
         cfd_model_dims(r.model, n, nullptr);
         r.estimate = estimate(r.cfg->cfg, n);
-        r.work = (double)n[0] * n[1] * n[2] * std::max(n[3], 1);
+        r.cells = (double)n[0] * n[1] * n[2];
+        r.work = r.cells * std::max(n[3], 1);
         return true;
     }
 
@@ -260,56 +264,90 @@ struct cfd_server {
         }
     }
 
-    // stepping thread, packed mode: join newly admitted requests to their
-    // precision's pack, run one quantum of every pack (as many steps as the
-    // member with the fewest left needs, at most opts.quantum, so no member
-    // idles), then check and release finished or failed members
-    cfd_pack *packs[2] = {nullptr, nullptr};  // FP32, FP64
-    void pack_quantum(const std::vector<Request *> &act) {
+    // Stepping thread.  Requests below opts.pack_cells cells are members of
+    // a pack of their precision: each pass of a step is one launch over all
+    // of them, whatever their sizes.  Other requests run their quanta on
+    // their own streams, beside the packs.  Every idle pack and stream
+    // request gets its next quantum (asynchronous); completed quanta are then
+    // checked (non-finite watchdog) and finished or failed requests handed to
+    // the host thread.
+    // A pack quantum has opts.quantum steps; a member whose run ends inside it
+    // idles for the rest (equal lengths let a stable membership replay
+    // captured sequences).  New members join between quanta.
+    // Two packs per precision, on their own streams, so that one pack's
+    // short passes (TF/SF, special cells, ports) overlap the other's updates;
+    // a new member joins the one with fewer cells.  Pack 2 w + h: precision w
+    // (0 FP32, 1 FP64), half h.
+    static constexpr int kPacks = 4;
+    cfd_pack *packs[kPacks] = {};
+    bool pack_busy[kPacks] = {};
+    double pack_load[kPacks] = {};           // member cells
+    std::vector<Request *> pack_run[kPacks];  // members of the quantum in flight
+
+    bool packed(const Request *r) const { return r->cells < opts.pack_cells; }
+
+    void launch_packs(const std::vector<Request *> &act) {
         for (Request *r : act) {
-            if (r->in_pack) continue;
-            cfd_pack *&p = packs[r->cfg->cfg.precision == CFD_F64 ? 1 : 0];
+            if (!packed(r) || r->in_pack) continue;
+            const int base = r->cfg->cfg.precision == CFD_F64 ? 2 : 0;
+            const int w = pack_load[base] <= pack_load[base + 1] ? base : base + 1;  // the lighter half
+            if (pack_busy[w]) continue;  // joins between quanta
+            cfd_pack *&p = packs[w];
             if ((!p && cfd_pack_create(r->cfg->cfg.precision, &p) < 0) || cfd_pack_add(p, r->sim) < 0) {
                 retire(r, cfd_last_error());
                 continue;
             }
             r->in_pack = true;
+            r->pack = w;
+            pack_load[w] += r->cells;
         }
-        std::vector<Request *> mem;
-        int q = opts.quantum;
-        for (Request *r : act)
-            if (r->in_pack) {
-                mem.push_back(r);
-                q = std::min(q, r->nt - r->done);
+        for (int w = 0; w < kPacks; w++) {
+            if (pack_busy[w] || !packs[w]) continue;
+            pack_run[w].clear();
+            for (Request *r : act)
+                if (r->in_pack && r->pack == w) pack_run[w].push_back(r);
+            if (pack_run[w].empty()) continue;
+            if (cfd_pack_step(packs[w], opts.quantum) < 0) {
+                const std::string e = cfd_last_error();
+                for (Request *r : pack_run[w]) {
+                    cfd_pack_remove(packs[w], r->sim);
+                    r->in_pack = false;
+                    pack_load[w] -= r->cells;
+                    retire(r, e);
+                }
+                continue;
             }
-        if (mem.empty()) {
-            std::this_thread::sleep_for(std::chrono::microseconds(200));
-            return;
+            pack_busy[w] = true;
         }
-        std::string err[2];
-        for (int w = 0; w < 2; w++)
-            if (packs[w] && cfd_pack_size(packs[w]) > 0 && cfd_pack_step(packs[w], q) < 0) err[w] = cfd_last_error();
-        for (int w = 0; w < 2; w++)
-            if (packs[w] && cfd_pack_size(packs[w]) > 0 && err[w].empty() && cfd_pack_sync(packs[w]) < 0)
-                err[w] = cfd_last_error();
-        for (Request *r : mem) {
-            const int w = r->cfg->cfg.precision == CFD_F64 ? 1 : 0;
-            r->done += q;
-            steps_launched += q;
-            std::string e = err[w];
-            if (e.empty()) {
-                // watchdog: a request whose fields went non-finite fails now,
-                // alone, instead of running on and returning garbage
-                const int bad = cfd_check_finite(r->sim);
-                if (bad < 0) e = cfd_last_error();
-                else if (bad > 0) e = "non-finite field values (" + std::to_string(bad) + ") by step " + std::to_string(r->done);
-            }
+    }
+    // a pack quantum completed (or failed with err): watchdog, step counts,
+    // finished and failed members leave the pack
+    void finish_pack_quantum(int w, std::string err) {
+        std::vector<Request *> &mem = pack_run[w];
+        std::vector<cfd_sim *> sims;
+        for (Request *r : mem) sims.push_back(r->sim);
+        std::vector<int> bad(sims.size(), 0);
+        // watchdog: a request whose fields went non-finite fails now, alone,
+        // instead of running on and returning garbage
+        if (err.empty() && cfd_pack_check_finite(packs[w], sims.data(), (int)sims.size(), bad.data()) < 0)
+            err = cfd_last_error();
+        for (size_t k = 0; k < mem.size(); k++) {
+            Request *r = mem[k];
+            const int ran = std::min(opts.quantum, r->nt - r->done);
+            r->done += ran;
+            steps_launched += ran;
+            std::string e = err;
+            if (e.empty() && bad[k] > 0)
+                e = "non-finite field values (" + std::to_string(bad[k]) + ") by step " + std::to_string(r->done);
             if (!e.empty() || r->done >= r->nt) {
                 cfd_pack_remove(packs[w], r->sim);
                 r->in_pack = false;
+                pack_load[w] -= r->cells;
                 retire(r, e);
             }
         }
+        mem.clear();
+        pack_busy[w] = false;
     }
 
     void step_loop() {
@@ -317,20 +355,20 @@ struct cfd_server {
             std::vector<Request *> act;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_step.wait(lk, [&] { return stop || !active.empty(); });
+                cv_step.wait(lk, [&] {
+                    return stop || !active.empty() || std::any_of(pack_busy, pack_busy + kPacks, [](bool b) { return b; });
+                });
                 if (stop) break;
                 act = active;
             }
-            if (!opts.streams) {
-                pack_quantum(act);
-                continue;
-            }
-            // every idle request gets its next quantum (asynchronous, own stream);
-            // a request retired here belongs to the host thread from then on
-            // (it may be finished and released at once), so the loop below
-            // must not touch it
+            launch_packs(act);
+            // every idle stream request gets its next quantum; a request
+            // retired here belongs to the host thread from then on (it may be
+            // finished and released at once), so the loop below must not
+            // touch it
             std::vector<Request *> stepping;
             for (Request *r : act) {
+                if (packed(r)) continue;
                 if (r->inflight) {
                     stepping.push_back(r);
                     continue;
@@ -346,14 +384,20 @@ struct cfd_server {
                 stepping.push_back(r);
             }
             bool any = false;
+            for (int w = 0; w < kPacks; w++) {
+                if (!pack_busy[w]) continue;
+                const int q = cfd_pack_query(packs[w]);
+                if (q == 0) continue;
+                any = true;
+                finish_pack_quantum(w, q < 0 ? std::string(cfd_last_error()) : std::string());
+            }
             for (Request *r : stepping) {
                 const int q = cfd_query(r->sim);
                 if (q == 0) continue;
                 any = true;
                 r->inflight = false;
                 if (q < 0) { retire(r, cfd_last_error()); continue; }
-                // watchdog: a request whose fields went non-finite fails now,
-                // alone, instead of running on and returning garbage
+                // watchdog, as for pack members
                 const int bad = cfd_check_finite(r->sim);
                 if (bad != 0) {
                     retire(r, bad < 0 ? std::string(cfd_last_error())
@@ -365,9 +409,15 @@ struct cfd_server {
             if (!any) std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
         // stopping: the packs let go of their members (shutdown fails them)
-        for (cfd_pack *&p : packs) {
-            if (p) cfd_pack_destroy(p);
-            p = nullptr;
+        for (int w = 0; w < kPacks; w++) {
+            if (packs[w]) {
+                cfd_pack_sync(packs[w]);
+                cfd_pack_destroy(packs[w]);
+            }
+            packs[w] = nullptr;
+            pack_busy[w] = false;
+            pack_load[w] = 0;
+            pack_run[w].clear();
         }
     }
     // stepping thread: hand a request over to the host thread
@@ -407,14 +457,18 @@ int cfd_server_create(const cfd_server_opts *o, cfd_server **out)
     if (!out) { g_server_error = "null argument"; return -1; }
     auto s = std::make_unique<cfd_server>();
     s->opts.hbm_budget = o && o->hbm_budget ? o->hbm_budget : (size_t)200 << 30;
-    // defaults measured on the c5 stream (MI355X, 120 requests): 4 running
-    // requests gave p50 21 s against 32 s with 64 and the same throughput (the
-    // GPU is saturated either way; fewer at once finish early arrivals sooner);
-    // 256-step quanta are two replayed 128-step graph blocks
+    // defaults measured on the first 120 requests of the mixed-size FP32
+    // stream (MI355X): one stream per request, 8 at once, 256-step quanta:
+    // 39.0 Gcell/s, p50 14.1 s, p99 24.5 s.  4 at once: 38.6; 64 at once:
+    // the same rate and p50 32 s (the GPU is saturated either way, fewer at
+    // once finish early arrivals sooner).  Packing requests below 2e6 cells
+    // (two packs per precision): 36.3, below 4e6 cells: 34.2 (the small
+    // grids' passes are CPML-bound in the kernels themselves; packing removes
+    // launches, which were not the limit).
     s->opts.max_active = o && o->max_active > 0 ? o->max_active : 8;
     s->opts.quantum = o && o->quantum > 0 ? o->quantum : 256;
     s->opts.policy = o && o->policy == CFD_POLICY_SHORTEST ? CFD_POLICY_SHORTEST : CFD_POLICY_FIFO;
-    s->opts.streams = o && o->streams ? 1 : 0;
+    s->opts.pack_cells = o && o->pack_cells > 0 ? o->pack_cells : 0;
     cfd_server *p = s.get();
     p->stepper = std::thread([p] { p->step_loop(); });
     p->host = std::thread([p] { p->host_loop(); });

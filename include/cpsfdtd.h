@@ -170,6 +170,9 @@ int cfd_pack_destroy(cfd_pack *p);                  /* members stay valid handle
 int cfd_pack_add(cfd_pack *p, cfd_sim *s);
 int cfd_pack_remove(cfd_pack *p, cfd_sim *s);
 int cfd_pack_size(cfd_pack *p);
+/* non-finite field values of n members (counts[q] for sims[q]; the serving
+ * loop's watchdog), counted after the pack's queued steps, one wait */
+int cfd_pack_check_finite(cfd_pack *p, cfd_sim *const *sims, int n, int *counts);
 int cfd_pack_step(cfd_pack *p, int count);          /* asynchronous; a finished member idles */
 int cfd_pack_sync(cfd_pack *p);
 int cfd_pack_query(cfd_pack *p);                    /* 1 done, 0 running */
@@ -177,12 +180,11 @@ int cfd_pack_set_timing(cfd_pack *p, int on);
 int cfd_pack_kernel_times(cfd_pack *p, char (*names)[32], double *ms, int *launches, int cap);
 
 /* Serving: requests (one cfd_config each) admitted first-come-first-served
- * while their device memory fits hbm_budget, then stepped in quanta of at
- * most `quantum` steps, all running requests of one precision together as
- * the members of a pack (one launch per pass over all of them; see
- * cfd_pack_*), or, with `streams` set, each on its own stream; a finished
- * request's port results (|E|^2 series, DFT)
- * are copied to host memory and its device memory freed.  A scheduler thread
+ * while their device memory fits hbm_budget, then stepped in quanta of
+ * `quantum` steps, each request on its own stream, or, below pack_cells
+ * cells, as a member of a pack (one launch per pass over all of them; see
+ * cfd_pack_*); a finished request's port results (|E|^2 series, DFT) are
+ * copied to host memory and its device memory freed.  A scheduler thread
  * runs inside the server; the calls below are thread-safe.  No reference
  * counterpart (the reference runs one simulation per process,
  * main.cpp:115-343). */
@@ -196,10 +198,10 @@ int cfd_pack_kernel_times(cfd_pack *p, char (*names)[32], double *ms, int *launc
 #define CFD_POLICY_SHORTEST 1   /* admit the queued request with the least work (cells x steps) */
 typedef struct {
     size_t hbm_budget;   /* bytes of device memory for running requests (0: 200 GiB) */
-    int max_active;      /* requests running at once (0: 4) */
+    int max_active;      /* requests running at once (0: 8) */
     int quantum;         /* steps per scheduling slice (0: 256) */
     int policy;          /* CFD_POLICY_* (0: FIFO) */
-    int streams;         /* 1: each request on its own stream instead of packed launches */
+    double pack_cells;   /* requests with fewer cells are packed (0: none) */
 } cfd_server_opts;
 typedef struct cfd_server cfd_server;
 int cfd_server_create(const cfd_server_opts *opts, cfd_server **out);

@@ -1,7 +1,7 @@
 """Packs (cfd_pack_*): members of different grid sizes and physics stepped
 together, one launch per pass over all of them, give the results each gives
 alone, bit for bit, in FP64 and FP32: fields, port samples, |E|^2 series,
-DFT accumulators and gain populations.  Runs cross replayed 128-step
+DFT accumulators and gain populations.  Runs cross replayed 16-step
 sequences and eager remainders; members end at different steps, and one joins
 a running pack."""
 import numpy as np
@@ -72,15 +72,16 @@ def test_pack_members_equal_solo_runs(tmp_path, precision):
     for name in ("dlgain", "periodic", "small"):
         pack.add(members[name][0])
     assert len(pack) == 3
-    pack.step(160)          # one replayed 128-step sequence and 32 eager steps; "small" ends at 140
+    pack.step(160)          # ten replayed 16-step sequences; "small" ends at 140
     pack.sync()
     assert members["small"][0].steps_done == 140
     pack.remove(members["small"][0])
     pack.add(members["c5gain"][0])  # joins at its step 0 while the others run on
-    pack.step(180)          # "periodic" ends at 260 inside this
+    pack.step(180)          # 11 replays and 4 eager steps; "periodic" ends at 260 inside this
     pack.sync()
     pack.step(200)
     pack.sync()
+    assert pack.check_finite() == [0] * len(pack)  # the serving watchdog's count
     for name, (e, case) in members.items():
         assert e.steps_done == case.nt, name
     pack.close()

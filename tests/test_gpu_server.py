@@ -1,6 +1,6 @@
 """The serving loop (cfd_server_*, cpsfdtd_amd.Server): requests of different
-shapes run together in step quanta, as members of one pack per precision
-(one launch per pass over all of them) or each on its own stream; each
+shapes run together in step quanta, each on its own stream or as members of
+packs (one launch per pass over all of them); each
 request's results must equal a solo run of the same case bit for bit (FP64
 and FP32), a failing request must not disturb the others, and the memory
 budget must hold requests back.  The first eight requests of the seeded c5
@@ -32,9 +32,9 @@ def make(root, k, params):
     return configs.c5(d, params)
 
 
-@pytest.mark.parametrize("streams", [False, True], ids=["packed", "streams"])
-def test_server_results_equal_solo_runs(tmp_path, streams):
-    srv = eng.Server(max_active=3, quantum=37, streams=streams)
+@pytest.mark.parametrize("pack_cells", [1e9, 0], ids=["packed", "streams"])
+def test_server_results_equal_solo_runs(tmp_path, pack_cells):
+    srv = eng.Server(max_active=3, quantum=37, pack_cells=pack_cells)
     cases, ids = [], []
     for k, (p, prec) in enumerate(SMALL):
         case = make(tmp_path, k, p)
@@ -168,7 +168,8 @@ def test_c5_first_requests_real_size(tmp_path):
     sizes (up to 239 x 313 x 210 cells, Drude-Lorentz, gain and dielectric
     members): the FP64 engine equals the oracle's first 200 steps bit for bit
     (fields, port DFT; |E|^2 sums within 1e-12), and served in FP32 to their
-    last step (up to 6954 steps) each result equals its solo run bit for bit."""
+    last step (up to 6954 steps; those under 2e6 cells packed, the others on
+    their own streams) each result equals its solo run bit for bit."""
     g = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c5_first8.npz"),
                      allow_pickle=False))
     steps = int(g["steps"][0])
@@ -190,7 +191,7 @@ def test_c5_first_requests_real_size(tmp_path):
         assert _sha(np.stack([d.real, d.imag], axis=-1)) == bytes(g["dft_sha256%d" % q]).decode(), q
         e.close()
     print("c5: serving the eight requests in FP32", flush=True)
-    srv = eng.Server()
+    srv = eng.Server(pack_cells=2e6)
     ids = [srv.submit(case, 32) for case in cases]
     for rid in ids:
         assert srv.wait(rid, 300.0) == eng.REQ_DONE, srv.error(rid)
