@@ -358,8 +358,10 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack
                 time.sleep(dt)
             ids.append(srv.submit(case, precision))
         lat, first, last = [], None, 0.0
-        for rid in ids:
-            st = srv.wait(rid, 900.0)
+        for k, rid in enumerate(ids):
+            if k % 100 == 0:
+                log("bench: serving, %d of %d requests done (%.0f s)" % (k, len(ids), time.perf_counter() - t0))
+            st = srv.wait(rid, 1800.0)
             if st != eng.REQ_DONE:
                 raise RuntimeError("request %d: %s" % (rid, srv.error(rid)))
             _, (ts, tb, td) = srv.poll(rid)
@@ -384,6 +386,46 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack
             "own streams" % pack_cells if pack_cells else "one stream per request"}
 
 
+def serving_c2(nreq=4, precision=32, steps=4000):
+    """``nreq`` copies of the 256^3 workload (the whole 4000-step run each)
+    submitted at once to the server with its defaults: served rate over the
+    makespan and latency from submission to port results on the host, beside
+    the same requests solved one after another on the engine alone."""
+    import cpsfdtd_amd as eng
+    from cpsfdtd_amd import configs
+    cases = [configs.c2(tempfile.mkdtemp(prefix="bench_c2s_"), nt=steps) for _ in range(nreq)]
+    cells = 256 ** 3
+    srv = eng.Server()
+    try:
+        t0 = time.perf_counter()
+        ids = [srv.submit(c, precision) for c in cases]
+        lat = []
+        for rid in ids:
+            if srv.wait(rid, 900.0) != eng.REQ_DONE:
+                raise RuntimeError("request %d: %s" % (rid, srv.error(rid)))
+            _, (ts, tb, td) = srv.poll(rid)
+            lat.append(td - ts)
+            srv.release(rid)
+        span = time.perf_counter() - t0
+        stats = srv.stats()
+    finally:
+        srv.close()
+    t0 = time.perf_counter()
+    for c in cases:
+        e = eng.Engine(c, precision)
+        e.step(c.nt)
+        e.port_dft(0)
+        e.close()
+    solo = time.perf_counter() - t0
+    return {"workload": "c2: %d copies of the 256^3 workload, %d steps each, FP%d, submitted at once" % (nreq, steps, precision),
+            "requests": nreq, "value": round(nreq * cells * steps / span / 1e6, 2), "unit": "Mcell-updates/s (served)",
+            "makespan_s": round(span, 2), "p50_s": round(float(np.percentile(lat, 50)), 3),
+            "max_s": round(max(lat), 3), "peak_active": stats["peak_active"],
+            "solo_sequential": round(nreq * cells * steps / solo / 1e6, 2),
+            "basis": "server: setup, stepping, results to host and release inside the makespan; solo: the same "
+                     "per request (Engine set-up, steps, DFT read back, close)"}
+
+
 def cpu_fifo_model(nreq, mcells_per_s):
     """Latencies of the same request stream solved one after another, first
     come first served, by a CPU at a fixed rate (the measured reference rate
@@ -401,7 +443,7 @@ def cpu_fifo_model(nreq, mcells_per_s):
             "basis": "sequential FIFO solve at the CPU baseline's measured rate (a model, not a run)"}
 
 
-def other_configs(steps=512, warmup=128):
+def other_configs(steps=512, warmup=128, names=("c1", "c3")):
     """The single-grid workloads beside c2, each in FP32 and FP64: c1 (128^3
     sphere, Ez source, line port recording every step) and c3 (192x192x64
     rod slab with four-level gain, periodic x/y, sphere port), ``steps``
@@ -411,6 +453,8 @@ def other_configs(steps=512, warmup=128):
     from cpsfdtd_amd import configs
     out = {}
     for name, make in (("c1", configs.c1), ("c3", configs.c3)):
+        if name not in names:
+            continue
         case = make(tempfile.mkdtemp(prefix="bench_%s_" % name), nt=warmup + steps + BLOCK)
         model = eng.Model(case)
         N = model.N
@@ -449,6 +493,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served c5 measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the c1 / c3 measurements")
     ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "serve", "serve-c2"],
+                    help="only this figure (for per-workload profiles): print its JSON and exit")
     ap.add_argument("--no-quality", action="store_true", help="skip the two whole 4000-step quality runs")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--copy-calib", action="store_true",
@@ -479,6 +525,13 @@ def main():
     from cpsfdtd_amd import configs
     info = eng.device_info()
     log("bench: device", info)
+    if args.only:
+        log("bench: only %s" % args.only)
+        fig = {"c1": lambda: other_configs(names=("c1",)), "c3": lambda: other_configs(names=("c3",)),
+               "c4": batched_c4, "serve": lambda: serving_c5(args.serve_requests),
+               "serve-c2": serving_c2}[args.only]()
+        print(json.dumps({"only": args.only, "result": fig, "device": info["name"]}), flush=True)
+        return
     warmup, steps, tsteps = whole_blocks(args.warmup), whole_blocks(args.steps), whole_blocks(args.timing_steps)
     nt = warmup + steps + tsteps
     d = tempfile.mkdtemp(prefix="bench_c2_")
@@ -575,6 +628,13 @@ def main():
             out["batched"] = {"error": str(ex)}
             log("bench: batched c4 failed: %s" % ex)
     if not args.no_serve:
+        log("bench: serving four 256^3 c2 requests")
+        try:
+            out["serving_c2"] = serving_c2()
+            log("bench: served c2 %.1f Mcell-upd/s" % out["serving_c2"]["value"])
+        except Exception as ex:
+            out["serving_c2"] = {"error": str(ex)}
+            log("bench: served c2 failed: %s" % ex)
         log("bench: serving the first %d c5 requests" % args.serve_requests)
         # the server's default admission (first come, first served), then
         # shortest work first on the same stream

@@ -57,6 +57,8 @@ SIGNATURES = {
     "cfd_cpml_e": ([_vp], C.c_int),
     "cfd_tfsf_h": ([_vp, C.c_int], C.c_int),
     "cfd_tfsf_e": ([_vp, C.c_int], C.c_int),
+    "cfd_tfsf_face_h": ([_vp, C.c_int, C.c_int], C.c_int),
+    "cfd_tfsf_face_e": ([_vp, C.c_int, C.c_int], C.c_int),
     "cfd_dl_e": ([_vp], C.c_int),
     "cfd_gain_pol": ([_vp], C.c_int),
     "cfd_gain_e": ([_vp], C.c_int),
@@ -76,6 +78,10 @@ SIGNATURES = {
     "cfd_port_dft": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_port_intensity": ([_vp, C.c_int, _dp], C.c_int),
     "cfd_gain_read": ([_vp, _dp], C.c_int),
+    "cfd_dl_state": ([_vp, _dp], C.c_int),
+    "cfd_gain_state": ([_vp, _dp], C.c_int),
+    "cfd_special_cells": ([_vp, C.c_int, _ip], C.c_int),
+    "cfd_tfsf_wave": ([_vp, _dp, _dp], C.c_int),
     "cfd_model_build": ([C.POINTER(_Config), C.POINTER(_vp)], C.c_int),
     "cfd_create_from_model": ([_vp, C.c_int, C.POINTER(_vp)], C.c_int),
     "cfd_model_free": ([_vp], C.c_int),
@@ -111,6 +117,7 @@ SIGNATURES = {
     "cfd_server_last_error": ([], C.c_char_p),
     "cfd_port_flux": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "cfd_dft_flux": ([_dp, C.c_int, C.c_int, C.c_int, C.c_double, _dp], C.c_int),
+    "cfd_tfsf_tables": ([C.c_char_p, C.c_double, C.c_int, _dp, _dp], C.c_int),
     "cfd_image_colormap": ([_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_materials": ([C.POINTER(C.c_short), C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_write_png": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
@@ -223,6 +230,11 @@ class Engine:
         self.shape = tuple(x + 1 for x in self.N)
         self.nports, self.n_dl, self.n_gain, self.n_tfsf = (int(x) for x in cnt)
         self.nf = len(case.freqs)
+        self.dl_per = 0  # values per component of a Drude-Lorentz cell: nD + 2 nL + 1
+        if case.files.get("dl"):
+            with open(case.files["dl"]) as f:
+                nD, nL = (int(f.readline().split()[0]) for _ in range(2))
+            self.dl_per = nD + 2 * nL + 1
 
     def close(self):
         if getattr(self, "h", None):
@@ -349,6 +361,35 @@ class Engine:
         a = np.zeros((self.n_gain, 4), np.float64)
         _check(library().cfd_gain_read(self.h, a.ctypes.data_as(_dp)))
         return a
+
+    def gain_state(self):
+        """[cell][6]: polarisation x y z, then its previous values."""
+        n = _check(library().cfd_gain_state(self.h, None))
+        a = np.zeros((max(n, 1), 6), np.float64)
+        _check(library().cfd_gain_state(self.h, a.ctypes.data_as(_dp)))
+        return a[:n]
+
+    def dl_state(self):
+        """[cell][3][nD + 2 nL + 1]: Drude currents, Lorentz currents, their
+        previous values, the previous field, per component."""
+        n = _check(library().cfd_dl_state(self.h, None))
+        a = np.zeros(max(n, 1) * 3 * self.dl_per, np.float64)
+        _check(library().cfd_dl_state(self.h, a.ctypes.data_as(_dp)))
+        return a[:n * 3 * self.dl_per].reshape(n, 3, self.dl_per)
+
+    def special_cells(self, kind):
+        """[cell][3] i j k of the Drude-Lorentz (0) or gain (1) cells."""
+        n = _check(library().cfd_special_cells(self.h, kind, None))
+        a = np.zeros((max(n, 1), 3), np.int32)
+        _check(library().cfd_special_cells(self.h, kind, a.ctypes.data_as(_ip)))
+        return a[:n]
+
+    def tfsf_wave(self):
+        """The incident tables Einc, Hinc."""
+        n = _check(library().cfd_tfsf_wave(self.h, None, None))
+        e, h = np.zeros(n), np.zeros(n)
+        _check(library().cfd_tfsf_wave(self.h, e.ctypes.data_as(_dp), h.ctypes.data_as(_dp)))
+        return e, h
 
     # -- timing ---------------------------------------------------------
     def set_timing(self, on=True):

@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -170,6 +171,7 @@ public:
     virtual void cpml_h() = 0;
     virtual void cpml_e() = 0;
     virtual void tfsf(bool h, int n) = 0;
+    virtual void tfsf_face(bool h, int face, int n) = 0;
     virtual void e_parts(unsigned parts) = 0;
     virtual void sources(int n) = 0;
     virtual void ports(int n) = 0;
@@ -185,6 +187,8 @@ public:
     virtual void port_dft(int q, double *host) = 0;
     virtual void port_intensity(int q, double *host) = 0;
     virtual void gain_read(double *host) = 0;
+    virtual int dl_state(double *host) = 0;
+    virtual int gain_state(double *host) = 0;
     virtual long long nonfinite() = 0;
 };
 
@@ -197,6 +201,10 @@ public:
     TfTarget *tfE = nullptr, *tfH = nullptr;
     int ntfE = 0, ntfH = 0, tablen = 0;
     double *tcoef = nullptr, *tdelay = nullptr, *tprof = nullptr, *einc = nullptr, *hinc = nullptr;
+    // one face alone, uploaded at its first use: targets [face][h], terms
+    TfTarget *tfFaceT[6][2] = {};
+    int ntfFace[6][2] = {};
+    double *fcoef = nullptr, *fdelay = nullptr, *fprof = nullptr;
     // sources, grouped by (cell, component)
     SourceGroup *sgroup = nullptr;
     int nsgroup = 0;
@@ -716,6 +724,36 @@ public:
         });
         HIP_OK(hipGetLastError());
     }
+    // TF_SF::{x,y,z}{1,2}Update{E,H}Field: the corrections of one face of
+    // the box (face = 2 * axis + high), each value corrected by one thread
+    void tfsf_face(bool h, int face, int n) override {
+        if (!m.has_tfsf) throw Error("no TF/SF set-up");
+        if (face < 0 || face > 5) throw Error("TF/SF face must be 0..5 (x1 x2 y1 y2 z1 z2)");
+        if (!fcoef) {
+            std::vector<double> c, dl, pr;
+            for (const TfsfTerm &t : m.tf_face_terms) { c.push_back(t.coef); dl.push_back(t.delay); pr.push_back(t.prof); }
+            for (int f = 0; f < 6; f++)
+                for (int hh = 0; hh < 2; hh++) {
+                    std::vector<TfTarget> tv;
+                    std::set<std::pair<int, long long>> seen;
+                    for (const TfsfTarget &t : m.tfFace[f][hh]) {
+                        tv.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
+                        if (!seen.insert({t.comp, tv.back().cell}).second) throw Error("TF/SF face lists a value twice");
+                    }
+                    ntfFace[f][hh] = (int)tv.size();
+                    tfFaceT[f][hh] = tv.empty() ? nullptr : upload<TfTarget>(tv);
+                }
+            fdelay = upload<double>(dl); fprof = upload<double>(pr);
+            fcoef = upload<double>(c);
+        }
+        const int nt = ntfFace[face][h];
+        if (!nt) return;
+        timer.run(stream, h ? "tfsf_face_h" : "tfsf_face_e", [&] {
+            hipLaunchKernelGGL((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, tfFaceT[face][h], nt,
+                               fcoef, fdelay, fprof, h ? einc : hinc, tablen, nullptr, n);
+        });
+        HIP_OK(hipGetLastError());
+    }
     void sources(int n) override {
         if (!nsgroup) return;
         if (!nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
@@ -928,6 +966,36 @@ public:
         HIP_OK(hipStreamSynchronize(stream));
         for (size_t r = 0; r < ng; r++)
             for (int l = 0; l < 4; l++) host[r * 4 + l] = tmp[l * ng + gain_slot[r]];
+    }
+    // device value (variable v, component c, slot s) at dlst[(v * 3 + c) * ndl + s]
+    int dl_state(double *host) override {
+        if (!m.has_dl) throw Error("no Drude-Lorentz medium");
+        const size_t nd = d.ndl, per = (size_t)m.nD + 2 * m.nL + 1;
+        if (host && nd) {
+            std::vector<R> tmp(per * 3 * nd);
+            HIP_OK(hipMemcpyAsync(tmp.data(), d.dlst, tmp.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            for (size_t r = 0; r < nd; r++)
+                for (size_t c = 0; c < 3; c++)
+                    for (size_t v = 0; v < per; v++) host[(r * 3 + c) * per + v] = (double)tmp[(v * 3 + c) * nd + dl_slot[r]];
+        }
+        return (int)nd;
+    }
+    int gain_state(double *host) override {
+        if (!m.has_gain) throw Error("no gain medium");
+        const size_t ng = d.ngain;
+        if (host && ng) {
+            std::vector<R> p(3 * ng), p1(3 * ng);
+            HIP_OK(hipMemcpyAsync(p.data(), d.gP, p.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipMemcpyAsync(p1.data(), d.gP1, p1.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            for (size_t r = 0; r < ng; r++)
+                for (size_t c = 0; c < 3; c++) {
+                    host[r * 6 + c] = (double)p[c * ng + gain_slot[r]];
+                    host[r * 6 + 3 + c] = (double)p1[c * ng + gain_slot[r]];
+                }
+        }
+        return (int)ng;
     }
 };
 
@@ -1580,6 +1648,8 @@ int cfd_cpml_h(cfd_sim *s) { NEED_SOLO(s); return guard([&] { s->impl->cpml_h();
 int cfd_cpml_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { s->impl->cpml_e(); }); }
 int cfd_tfsf_h(cfd_sim *s, int n) { NEED_SOLO(s); return guard([&] { s->impl->tfsf(true, n); }); }
 int cfd_tfsf_e(cfd_sim *s, int n) { NEED_SOLO(s); return guard([&] { s->impl->tfsf(false, n); }); }
+int cfd_tfsf_face_h(cfd_sim *s, int face, int n) { NEED_SOLO(s); return guard([&] { s->impl->tfsf_face(true, face, n); }); }
+int cfd_tfsf_face_e(cfd_sim *s, int face, int n) { NEED_SOLO(s); return guard([&] { s->impl->tfsf_face(false, face, n); }); }
 int cfd_dl_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_dl) s->impl->e_parts(cfd::E_DL); }); }
 int cfd_gain_pol(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINPOL); }); }
 int cfd_gain_e(cfd_sim *s) { NEED_SOLO(s); return guard([&] { if (s->impl->m.has_gain) s->impl->e_parts(cfd::E_GAINE); }); }
@@ -1672,6 +1742,52 @@ int cfd_port_flux(cfd_sim *s, int port, int axis, double *host)
     });
 }
 int cfd_gain_read(cfd_sim *s, double *host) { NEED(s); return guard([&] { s->impl->gain_read(host); }); }
+int cfd_dl_state(cfd_sim *s, double *host)
+{
+    NEED(s);
+    int n = 0;
+    const int rc = guard([&] { n = s->impl->dl_state(host); });
+    return rc < 0 ? rc : n;
+}
+int cfd_gain_state(cfd_sim *s, double *host)
+{
+    NEED(s);
+    int n = 0;
+    const int rc = guard([&] { n = s->impl->gain_state(host); });
+    return rc < 0 ? rc : n;
+}
+int cfd_special_cells(cfd_sim *s, int kind, int *ijk)
+{
+    NEED(s);
+    int n = 0;
+    const int rc = guard([&] {
+        const cfd::Model &m = s->impl->m;
+        if (kind != 0 && kind != 1) throw cfd::Error("kind must be 0 (Drude-Lorentz) or 1 (gain)");
+        const std::vector<long long> &cells = kind ? m.gain_cells : m.dl_cells;
+        n = (int)cells.size();
+        if (ijk)
+            for (size_t r = 0; r < cells.size(); r++) {
+                const long long p = cells[r], kk = p % (m.Nz + 1), jj = (p / (m.Nz + 1)) % (m.Ny + 1);
+                ijk[3 * r] = (int)(p / ((long long)(m.Ny + 1) * (m.Nz + 1)));
+                ijk[3 * r + 1] = (int)jj;
+                ijk[3 * r + 2] = (int)kk;
+            }
+    });
+    return rc < 0 ? rc : n;
+}
+int cfd_tfsf_wave(cfd_sim *s, double *einc, double *hinc)
+{
+    NEED(s);
+    int n = 0;
+    const int rc = guard([&] {
+        const cfd::Model &m = s->impl->m;
+        if (!m.has_tfsf) throw cfd::Error("no TF/SF set-up");
+        n = (int)m.Einc.size();
+        if (einc) std::copy(m.Einc.begin(), m.Einc.end(), einc);
+        if (hinc) std::copy(m.Hinc.begin(), m.Hinc.end(), hinc);
+    });
+    return rc < 0 ? rc : n;
+}
 
 int cfd_set_timing(cfd_sim *s, int on)
 {

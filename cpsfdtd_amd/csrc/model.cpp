@@ -328,6 +328,28 @@ void tfsf_waveforms(Model &m, double lambda, double dlambda, int cwmode, int upt
     }
 }
 
+// the incident tables of a TF/SF file alone (TF_SF.cpp:33-89 lines 1, 2, 10,
+// 11 and :898-955): entries 0 .. max(Nt, nt) + 2
+void tfsf_file_tables(const char *path, double dt, int nt, std::vector<double> &einc, std::vector<double> &hinc) {
+    if (!(dt > 0)) throw Error("TF/SF tables need a time step > 0");
+    LineReader L(path);
+    double lambda, dlambda;
+    int Nt = 0, cw = 0;
+    L.read("lambda", lambda);
+    L.read("dlambda", dlambda);
+    for (int q = 0; q < 7; q++) L.next();
+    L.read("Nt", Nt);
+    {
+        std::istringstream is = L.next();
+        is >> cw;
+    }
+    Model t;
+    t.dt = dt;
+    tfsf_waveforms(t, lambda, dlambda, cw, std::max(Nt, nt));
+    einc = std::move(t.Einc);
+    hinc = std::move(t.Hinc);
+}
+
 // TF_SF.cpp:29-959 (geometry, tables) and :988-1301 (which component each face
 // record corrects, with which sign, in which order)
 void load_tfsf(Model &m, const char *path, int nt_req) {
@@ -415,6 +437,13 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
         }
         terms[t].push_back({coef, delay, prof});
     };
+    // a face alone: its records' corrections, each target once; the z1
+    // variants leave out the transverse profile (TF_SF.cpp:1126-1143 and
+    // :1375-1392 comment the factor out), the other faces keep it
+    auto add_face = [&](bool h, int face, int comp, int i, int j, int k, double coef, double delay, double prof) {
+        m.tfFace[face][h].push_back({comp, i, j, k, (int)m.tf_face_terms.size(), 1});
+        m.tf_face_terms.push_back({coef, delay, face == 4 ? 1.0 : prof});
+    };
     const int order[3] = {1, 0, 2};  // y faces, x faces, z faces
     for (int oa = 0; oa < 3; oa++) {
         const int a = order[oa];
@@ -460,9 +489,12 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
                     at(xe, ye, ze, dE, pE);
                     at(x, y, z, dH, pH);
                     const double s = sgn[a][c] * flip;
-                    add(false, compE[a][c], r.i, r.j, r.k, s < 0 ? -ampE[a][c] : ampE[a][c], dE, pE);
+                    const double cE = s < 0 ? -ampE[a][c] : ampE[a][c], cH = s < 0 ? -ampH[a][c] : ampH[a][c];
+                    add(false, compE[a][c], r.i, r.j, r.k, cE, dE, pE);
                     const double ph = ((m.flags & CFD_REF_ZEROHEAP) && a == 0 && r.part != 3) ? 0.0 : pH;
-                    add(true, compH[a][c], hi, hj, hk, s < 0 ? -ampH[a][c] : ampH[a][c], dH, ph);
+                    add(true, compH[a][c], hi, hj, hk, cH, dH, ph);
+                    add_face(false, 2 * a + e, compE[a][c], r.i, r.j, r.k, cE, dE, pE);
+                    add_face(true, 2 * a + e, compH[a][c], hi, hj, hk, cH, dH, ph);
                 }
             }
         }
@@ -635,6 +667,10 @@ void cell_lists(Model &m) {
 }
 
 }  // namespace
+
+void tfsf_tables(const char *path, double dt, int nt, std::vector<double> &einc, std::vector<double> &hinc) {
+    tfsf_file_tables(path, dt, nt, einc, hinc);
+}
 
 void check_config(const cfd_config &cfg) {
     if (!cfg.material || !cfg.structure) throw Error("material and structure files are required");

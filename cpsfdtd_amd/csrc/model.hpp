@@ -70,6 +70,11 @@ struct Model {
     std::vector<TfsfTarget> tfE, tfH;
     std::vector<TfsfTerm> tf_terms;
     std::vector<double> Einc, Hinc;  // nt + 3 entries
+    // one face alone (TF_SF::x1UpdateEField ... z2UpdateHField), [face][h],
+    // face = 2 * axis + (high face): one target per corrected value, terms
+    // in tf_face_terms
+    std::vector<TfsfTarget> tfFace[6][2];
+    std::vector<TfsfTerm> tf_face_terms;
 
     // Drude-Lorentz (cells with ID == 2)
     bool has_dl = false;
@@ -105,6 +110,8 @@ struct Model {
 // Throws Error naming the first problem.
 void check_config(const cfd_config &cfg);
 Model build_model(const cfd_config &cfg);
+// a TF/SF file's incident tables Einc, Hinc (entries 0 .. max(Nt, nt) + 2)
+void tfsf_tables(const char *path, double dt, int nt, std::vector<double> &einc, std::vector<double> &hinc);
 
 }  // namespace cfd
 

@@ -138,6 +138,20 @@ int cfd_model_cells(cfd_model *p, int gain, long long *flat)
     return (int)v.size();
 }
 
+int cfd_tfsf_tables(const char *tfsf, double dt, int nt, double *einc, double *hinc)
+{
+    try {
+        std::vector<double> e, h;
+        cfd::tfsf_tables(tfsf, dt, nt, e, h);
+        if (einc) std::memcpy(einc, e.data(), e.size() * sizeof(double));
+        if (hinc) std::memcpy(hinc, h.data(), h.size() * sizeof(double));
+        return (int)e.size();
+    } catch (const std::exception &ex) {
+        g_model_error = ex.what();
+        return -1;
+    }
+}
+
 int cfd_dft_flux(const double *dft, int npts, int nf, int axis, double scale, double *out)
 {
     if (!dft || !out || npts < 0 || nf < 0 || axis < 0 || axis > 2) {

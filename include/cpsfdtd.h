@@ -92,6 +92,18 @@ int cfd_cpml_h(cfd_sim *s);             /* CPML::UpdateCPMLHFields CPML.cpp:609-
 int cfd_cpml_e(cfd_sim *s);             /* CPML::UpdateCPMLEFields CPML.cpp:529-607     */
 int cfd_tfsf_h(cfd_sim *s, int n);      /* TF_SF::UpdateHField    TF_SF.cpp:1201-1301   */
 int cfd_tfsf_e(cfd_sim *s, int n);      /* TF_SF::UpdateEField    TF_SF.cpp:988-1089    */
+/* One face of the TF/SF box alone: TF_SF::x1UpdateHField ... z2UpdateHField
+ * (TF_SF.cpp:1302-1410) and x1UpdateEField ... z2UpdateEField
+ * (TF_SF.cpp:1090-1199).  The z1 forms leave out the transverse profile, as
+ * the reference's do. */
+#define CFD_FACE_X1 0
+#define CFD_FACE_X2 1
+#define CFD_FACE_Y1 2
+#define CFD_FACE_Y2 3
+#define CFD_FACE_Z1 4
+#define CFD_FACE_Z2 5
+int cfd_tfsf_face_h(cfd_sim *s, int face, int n);
+int cfd_tfsf_face_e(cfd_sim *s, int face, int n);
 int cfd_dl_e(cfd_sim *s);               /* Drude_Lorentz::EFieldUpdate Drude_Lorentz.cpp:153-227 (3-D) */
 int cfd_gain_pol(cfd_sim *s);           /* FourLevelLaser::UpdatePolarizationDensities FourLevelLaser.cpp:270-298 */
 int cfd_gain_e(cfd_sim *s);             /* FourLevelLaser::UpdateElectricField FourLevelLaser.cpp:300-308 */
@@ -136,6 +148,21 @@ int cfd_port_flux(cfd_sim *s, int port, int axis, double *host);
  * E_b H_a*); scale = 2 (stride dt)^2 dA gives J/Hz.  No device needed. */
 int cfd_dft_flux(const double *dft, int npts, int nf, int axis, double scale, double *out);
 int cfd_gain_read(cfd_sim *s, double *host);                    /* [cells][4] N0..N3 */
+/* Dispersive and gain state, cells in the reference's order (returns the
+ * cell count; host may be NULL):
+ * Drude-Lorentz, [cell][3 components][nD + 2 nL + 1]: the Drude currents,
+ * the Lorentz currents, their previous values, the previous field
+ * (DL_material JD*, JL*, JL*_1, E*_1, Drude_Lorentz.h:22-27, in 3-D);
+ * gain, [cell][6]: polarisation x y z, then its previous values (Pa*,
+ * Pa*_1 of FourLevelLaser.h:34,46; one transition in this 3-D model). */
+int cfd_dl_state(cfd_sim *s, double *host);
+int cfd_gain_state(cfd_sim *s, double *host);
+/* i j k of each Drude-Lorentz (kind 0) or gain (kind 1) cell: DLM[].ix/jy
+ * (Drude_Lorentz.h:25) and FourLevelLaser::location (FourLevelLaser.h:31). */
+int cfd_special_cells(cfd_sim *s, int kind, int *ijk);
+/* TF_SF::Einc and ::Hinc (TF_SF.h:57, filled by TF_SF.cpp:898-955); returns
+ * the table length (einc, hinc may be NULL). */
+int cfd_tfsf_wave(cfd_sim *s, double *einc, double *hinc);
 
 /* Batches: n simulations of one shape (grid, CPML, periodic axes, TF/SF box
  * and waveform, ports, run length, precision) whose media, geometry and
@@ -242,6 +269,11 @@ int cfd_model_table(cfd_model *m, const char *name, double *out);
  * receives the number of terms.  h = 0 for E corrections, 1 for H. */
 int cfd_model_tfsf(cfd_model *m, int h, int *targets, double *terms, int *nterms);
 int cfd_model_port(cfd_model *m, int port, int *ijk);               /* returns point count */
+/* A TF/SF file's incident tables alone, without a grid (TF_SF.cpp:898-955
+ * with the file's lambda, dlambda, Nt and mode lines): Einc[n], Hinc[n] for
+ * n = 0 .. max(Nt, nt) + 2 at time step dt; returns that length (einc, hinc
+ * may be NULL), -1 with the message in cfd_model_last_error. */
+int cfd_tfsf_tables(const char *tfsf, double dt, int nt, double *einc, double *hinc);
 /* DL / gain cell lists in reference [i][j][k] flat order; returns count */
 int cfd_model_cells(cfd_model *m, int gain, long long *flat);
 

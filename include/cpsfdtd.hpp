@@ -76,6 +76,26 @@ inline void check(int rc, const char *what)
     if (rc < 0) throw Error(std::string(what) + ": " + cfd_last_error());
 }
 
+// a parameter file read line by line, one istringstream per line, as the
+// reference's constructors read theirs (getline + istringstream)
+class Lines {
+public:
+    Lines(const char *who, const char *fileName)
+    {
+        std::ifstream in(fileName);
+        if (!in) throw Error(std::string(who) + ": cannot open " + fileName);
+        std::string l;
+        while (std::getline(in, l)) lines_.push_back(l);
+    }
+    std::istringstream next() { return std::istringstream(n_ < lines_.size() ? lines_[n_++] : std::string()); }
+private:
+    std::vector<std::string> lines_;
+    size_t n_ = 0;
+};
+
+// the reference's Constants.h values
+constexpr double kPi = 3.1415926535897932384626433832795, kC = 2.99792458E8;
+
 // One run: the files the objects registered, the options, and the device
 // simulation once it exists.
 struct Context {
@@ -358,7 +378,21 @@ public:
     {
         ctx->require_unstarted("CFS_PML");
         ctx->pml = fileName;
+        // the public parameters as CFS_PML.cpp:30-71 reads them
+        detail::Lines L("CFS_PML", fileName);
+        L.next() >> Npmlx1 >> Npmlx2;
+        L.next() >> Npmly1 >> Npmly2;
+        L.next() >> Npmlz1 >> Npmlz2;
+        L.next() >> p_order >> malp;
+        L.next() >> kapmax >> sig_ratio;
+        L.next() >> alpmax;
     }
+    // CFS_PML.h:37-39.  sigmax is per axis here (0.8 (m + 1) / (d eta0) x
+    // sig_ratio, CPML.cpp:361-519) and reported for none; alpmin is never read
+    // by the reference.  Num_of_PML_points has no counterpart: psi lives in
+    // per-slab arrays, not a point list.
+    double p_order = 0, kapmax = 0, alpmin = 0, alpmax = 0, malp = 0, sig_ratio = 0, sigmax = 0;
+    int Npmlx1 = 0, Npmly1 = 0, Npmlz1 = 0, Npmlx2 = 0, Npmly2 = 0, Npmlz2 = 0;
     void Initialize_PML(Structure &) {}
     void UpdatePMLEFields(Field &) { ctx->device([](cfd_sim *s) { return cfd_cpml_e(s); }, "UpdatePMLEFields"); }
     void UpdatePMLHFields(Field &) { ctx->device([](cfd_sim *s) { return cfd_cpml_h(s); }, "UpdatePMLHFields"); }
@@ -434,6 +468,9 @@ public:
         nu = C / lambda;
         T = 1.0 / nu;
         if (sS.dt > 0) nT = (int)(T / sS.dt);
+        file = fileName;
+        dt = sS.dt;
+        if (dt > 0) tables(std::max(Nt, ctx->nt));
     }
 
     // the reference's public parameters; angles in radians.  z0 is line 3's
@@ -455,21 +492,182 @@ public:
     {
         ctx->device([timestep](cfd_sim *s) { return cfd_tfsf_h(s, timestep); }, "TF_SF::UpdateHField");
     }
+    // one face of the box alone (TF_SF.cpp:1090-1199, 1302-1410); the z1
+    // forms leave the transverse profile out, as the reference's do
+    void x1UpdateEField(int n, Field &, Structure &) { face(false, CFD_FACE_X1, n); }
+    void x2UpdateEField(int n, Field &, Structure &) { face(false, CFD_FACE_X2, n); }
+    void y1UpdateEField(int n, Field &, Structure &) { face(false, CFD_FACE_Y1, n); }
+    void y2UpdateEField(int n, Field &, Structure &) { face(false, CFD_FACE_Y2, n); }
+    void z1UpdateEField(int n, Field &, Structure &) { face(false, CFD_FACE_Z1, n); }
+    void z2UpdateEField(int n, Field &, Structure &) { face(false, CFD_FACE_Z2, n); }
+    void x1UpdateHField(int n, Field &, Structure &) { face(true, CFD_FACE_X1, n); }
+    void x2UpdateHField(int n, Field &, Structure &) { face(true, CFD_FACE_X2, n); }
+    void y1UpdateHField(int n, Field &, Structure &) { face(true, CFD_FACE_Y1, n); }
+    void y2UpdateHField(int n, Field &, Structure &) { face(true, CFD_FACE_Y2, n); }
+    void z1UpdateHField(int n, Field &, Structure &) { face(true, CFD_FACE_Z1, n); }
+    void z2UpdateHField(int n, Field &, Structure &) { face(true, CFD_FACE_Z2, n); }
+
+    // TF_SF.cpp:966-986: the incident E (H) wave at a retarded time index,
+    // linear between table entries, 0 before the wave starts.  The tables
+    // (Einc, Hinc, TF_SF.h:57) are computed on the host from the file and the
+    // Structure's dt and grow past Nt on demand.
+    double Einc_onPoint(double retarded) { return on_point(Einc, retarded); }
+    double Hinc_onPoint(double retarded) { return on_point(Hinc, retarded); }
+    std::vector<double> Einc, Hinc;
 private:
+    void face(bool h, int f, int n)
+    {
+        ctx->device([h, f, n](cfd_sim *s) { return h ? cfd_tfsf_face_h(s, f, n) : cfd_tfsf_face_e(s, f, n); },
+                    h ? "TF_SF::*UpdateHField (one face)" : "TF_SF::*UpdateEField (one face)");
+    }
+    void tables(int upto)
+    {
+        const int len = cfd_tfsf_tables(file.c_str(), dt, upto, nullptr, nullptr);
+        if (len < 0) throw Error(std::string("TF_SF tables: ") + cfd_model_last_error());
+        Einc.assign((size_t)len, 0.0);
+        Hinc.assign((size_t)len, 0.0);
+        cfd_tfsf_tables(file.c_str(), dt, upto, Einc.data(), Hinc.data());
+    }
+    double on_point(std::vector<double> &tab, double retarded)
+    {
+        if (retarded <= 0) return 0.0;
+        const int ntemp = (int)std::floor(retarded);
+        const double resid = retarded - (double)ntemp;
+        if ((size_t)ntemp + 1 >= tab.size()) {
+            if (!(dt > 0)) throw Error("TF_SF::*inc_onPoint: construct TF_SF after Structure::generate_mesh");
+            tables(std::max(ntemp + 1, ctx->nt));
+        }
+        return ((1.0 - resid) * tab[ntemp] + resid * tab[ntemp + 1]);
+    }
+    std::string file;
+    double dt = 0;
     std::shared_ptr<detail::Context> ctx;
+};
+
+// ---------------------------------------------------------------------------
+// Source (Sources.h:24-40): the point-source waveforms of a source file
+// (line 1 lambda, 2 FWHM, 3 delay; Sources.cpp:29-57), host only
+// ---------------------------------------------------------------------------
+class Source {
+public:
+    Source(double deltat, const char *fileName) : dt(deltat)
+    {
+        detail::Lines L("Source", fileName);
+        L.next() >> lambda;
+        L.next() >> FWHM;
+        L.next() >> delay;
+        nu = detail::kC / lambda;
+        T = 1.0 / nu;
+        nT = (int)(T / dt);
+        pinudt = 2.0 * detail::kPi * nu * dt;
+        nw = 2 * nT;
+        nd = 4 * nw;
+        nTrans = nd + nw;
+    }
+    // Sources.cpp:71-76: a Gaussian-windowed sine centred nd steps after the
+    // retarded time origin (ntimestep - location / (C dt)); FWHM and delay
+    // are read but not used, as in the reference
+    double Pulse(double Amplitude, double ntimestep, double location)
+    {
+        retard = ntimestep - location / detail::kC / dt;
+        const double x = retard - nd;
+        return Amplitude * std::exp(-x * x / (double)(nw * nw)) * std::sin(pinudt * x);
+    }
+    // Sources.cpp:60-69: the same window as a ramp up to retarded time nd,
+    // then the steady sine
+    double ContinuousWave(double Amplitude, double ntimestep, double location)
+    {
+        retard = ntimestep - location / detail::kC / dt;
+        if (retard < nd) {
+            const double x = retard - nd;
+            return Amplitude * std::exp(-x * x / (double)(nw * nw)) * std::sin(pinudt * retard);
+        }
+        return Amplitude * std::sin(pinudt * retard);
+    }
+    double lambda = 0, nu = 0, T = 0, delay = 0, FWHM = 0;
+private:
+    double dt = 0, retard = 0, pinudt = 0;
+    int nd = 0, nw = 0, nT = 0, nTrans = 0;
 };
 
 // ---------------------------------------------------------------------------
 // Drude_Lorentz (Drude_Lorentz.h:28-45), 3-D, on ID 2 cells
 // ---------------------------------------------------------------------------
+// one Drude-Lorentz cell's state (DL_material, Drude_Lorentz.h:22-27, in
+// 3-D): per component the Drude currents and the Lorentz currents with their
+// previous values, one value per pole, and the previous field
+struct DL_material {
+    int ix = 0, jy = 0, kz = 0;
+    std::vector<double> JDx, JDy, JDz, JLx, JLy, JLz, JLx_1, JLy_1, JLz_1;
+    double Ex_1 = 0, Ey_1 = 0, Ez_1 = 0;
+};
+
 class Drude_Lorentz {
 public:
     Drude_Lorentz(Structure &sS, const char *fileName) : ctx(sS.ctx)
     {
         ctx->require_unstarted("Drude_Lorentz");
         ctx->dl = fileName;
+        // the public parameters (Drude_Lorentz.h:33-35): pole counts,
+        // eps_inf, then one line per Drude pole (omega, gamma) and per
+        // Lorentz pole (omega, delta, delta eps); sigma is 0 (F10)
+        detail::Lines L("Drude_Lorentz", fileName);
+        L.next() >> num_of_D_poles;
+        L.next() >> num_of_L_poles;
+        L.next() >> epsInf;
+        for (int q = 0; q < num_of_D_poles; q++) {
+            double w = 0, g = 0;
+            L.next() >> w >> g;
+            omega_D.push_back(w);
+            gamma_D.push_back(g);
+        }
+        for (int q = 0; q < num_of_L_poles; q++) {
+            double w = 0, d = 0, de = 0;
+            L.next() >> w >> d >> de;
+            omega_L.push_back(w);
+            delta_L.push_back(d);
+            delta_eps_L.push_back(de);
+        }
     }
     void EFieldUpdate(Field &, Structure &) { ctx->device([](cfd_sim *s) { return cfd_dl_e(s); }, "Drude_Lorentz::EFieldUpdate"); }
+
+    double sigma = 0, epsInf = 0;
+    std::vector<double> omega_D, gamma_D, delta_eps_L, omega_L, delta_L;
+    int num_of_D_poles = 0, num_of_L_poles = 0;
+
+    // device state, read back (the simulation starts if it has not)
+    int num_of_DLM_gridcells()
+    {
+        const int n = cfd_special_cells(ctx->get(), 0, nullptr);
+        detail::check(n, "cfd_special_cells");
+        return n;
+    }
+    std::vector<DL_material> DLM()
+    {
+        cfd_sim *s = ctx->get();
+        const int n = num_of_DLM_gridcells();
+        std::vector<int> ijk((size_t)n * 3);
+        detail::check(cfd_special_cells(s, 0, ijk.data()), "cfd_special_cells");
+        const int nD = num_of_D_poles, nL = num_of_L_poles, per = nD + 2 * nL + 1;
+        std::vector<double> v((size_t)n * 3 * per);
+        detail::check(cfd_dl_state(s, v.data()), "cfd_dl_state");
+        std::vector<DL_material> out((size_t)n);
+        for (int r = 0; r < n; r++) {
+            DL_material &M = out[(size_t)r];
+            M.ix = ijk[3 * r]; M.jy = ijk[3 * r + 1]; M.kz = ijk[3 * r + 2];
+            std::vector<double> *JD[3] = {&M.JDx, &M.JDy, &M.JDz}, *JL[3] = {&M.JLx, &M.JLy, &M.JLz},
+                                *JL1[3] = {&M.JLx_1, &M.JLy_1, &M.JLz_1};
+            double *E1[3] = {&M.Ex_1, &M.Ey_1, &M.Ez_1};
+            for (int c = 0; c < 3; c++) {
+                const double *w = v.data() + ((size_t)r * 3 + c) * per;
+                JD[c]->assign(w, w + nD);
+                JL[c]->assign(w + nD, w + nD + nL);
+                JL1[c]->assign(w + nD + nL, w + nD + 2 * nL);
+                *E1[c] = w[nD + 2 * nL];
+            }
+        }
+        return out;
+    }
 private:
     std::shared_ptr<detail::Context> ctx;
 };
@@ -483,6 +681,68 @@ public:
     {
         ctx->require_unstarted("FourLevelLaser");
         ctx->gain = fileName;
+        // the public parameters as FourLevelLaser.cpp:37-97 reads them (line
+        // 15, the pump rate, is this engine's; F14)
+        detail::Lines L("FourLevelLaser", fileName);
+        std::string d;
+        L.next() >> d;
+        std::snprintf(distri, sizeof distri, "%s", d.c_str());
+        L.next() >> NumSources;
+        L.next() >> i1 >> i2;
+        L.next() >> j1 >> j2;
+        L.next() >> lambda_b;
+        L.next() >> lambda_a;
+        L.next() >> gamma_a;
+        L.next() >> gamma_b;
+        L.next() >> t30;
+        L.next() >> t32;
+        L.next() >> t21;
+        L.next() >> t10;
+        L.next() >> N_0;
+        L.next() >> N_1;
+        L.next() >> pump_rate;
+        omega_a = 2.0 * detail::kPi * detail::kC / lambda_a;
+        omega_b = 2.0 * detail::kPi * detail::kC / lambda_b;
+        idif = i2 - i1;
+        jdif = j2 - j1;
+    }
+    // FourLevelLaser.h:31-38.  NumSources, i1..jdif and distri are the file's
+    // values; the gain cells themselves are the ID 5 cells (location()).
+    int NumSources = 0;
+    int i1 = 0, i2 = 0, j1 = 0, j2 = 0, idif = 0, jdif = 0;
+    double N_0 = 0, N_1 = 0;
+    double gamma_a = 0, gamma_b = 0, lambda_a = 0, lambda_b = 0, omega_a = 0, omega_b = 0;
+    double t30 = 0, t32 = 0, t21 = 0, t10 = 0, pump_rate = 0;
+    char distri[40] = {0};
+
+    // device state, read back (the simulation starts if it has not):
+    // i j k of each gain cell (location), populations N0..N3 and the
+    // polarisation Pax Pay Paz (one transition in this 3-D model)
+    std::vector<int> location()
+    {
+        cfd_sim *s = ctx->get();
+        const int n = cfd_special_cells(s, 1, nullptr);
+        detail::check(n, "cfd_special_cells");
+        std::vector<int> ijk((size_t)n * 3);
+        detail::check(cfd_special_cells(s, 1, ijk.data()), "cfd_special_cells");
+        return ijk;
+    }
+    std::vector<double> N0() { return level(0); }
+    std::vector<double> N1() { return level(1); }
+    std::vector<double> N2() { return level(2); }
+    std::vector<double> N3() { return level(3); }
+    std::vector<double> Pax() { return pol(0); }
+    std::vector<double> Pay() { return pol(1); }
+    std::vector<double> Paz() { return pol(2); }
+    // [cell][6]: polarisation x y z, then its previous values
+    std::vector<double> polarization()
+    {
+        cfd_sim *s = ctx->get();
+        const int n = cfd_gain_state(s, nullptr);
+        detail::check(n, "cfd_gain_state");
+        std::vector<double> v((size_t)n * 6);
+        detail::check(cfd_gain_state(s, v.data()), "cfd_gain_state");
+        return v;
     }
     void UpdatePolarizationDensities(Field &) { ctx->device([](cfd_sim *s) { return cfd_gain_pol(s); }, "UpdatePolarizationDensities"); }
     void UpdateElectricField(Field &) { ctx->device([](cfd_sim *s) { return cfd_gain_e(s); }, "UpdateElectricField"); }
@@ -497,6 +757,20 @@ public:
         return v;
     }
 private:
+    std::vector<double> level(int l)
+    {
+        const std::vector<double> p = populations();
+        std::vector<double> v;
+        for (size_t r = 0; r < p.size() / 4; r++) v.push_back(p[r * 4 + l]);
+        return v;
+    }
+    std::vector<double> pol(int c)
+    {
+        const std::vector<double> p = polarization();
+        std::vector<double> v;
+        for (size_t r = 0; r < p.size() / 6; r++) v.push_back(p[r * 6 + c]);
+        return v;
+    }
     std::shared_ptr<detail::Context> ctx;
 };
 

@@ -182,6 +182,7 @@ struct orc_sim {
     int nports; port_t *port; int nf; double *freqs; int stride, record;
     int nt, ndone;
     double *twE, *twH; /* [nf][nt] cos/sin pairs */
+    int nfaces, faces[64]; /* TF/SF face variants in place of the whole update */
 };
 
 enum { T_cExy, T_cEyx, T_cEzx, T_cExz, T_cEzy, T_cEyz, T_cHxy, T_cHyx, T_cHzx, T_cHxz, T_cHzy, T_cHyz };
@@ -994,7 +995,7 @@ static double interp(const double *tab, double retarded)
  * tangential components the face's main loop corrects (edge loops correct
  * one of them); the sign is the low face's, the high face flips it.
  *   E: TF_SF.cpp:991-1087   H: TF_SF.cpp:1204-1300 */
-static void tfsf_apply(orc_sim *s, int n, int is_h)
+static void tfsf_face_apply(orc_sim *s, int n, int is_h, int a, int e, int variant)
 {
     tfsf_t *T = &s->tf;
     double *Ex = s->F[0], *Ey = s->F[1], *Ez = s->F[2], *Hx = s->F[3], *Hy = s->F[4], *Hz = s->F[5];
@@ -1004,11 +1005,10 @@ static void tfsf_apply(orc_sim *s, int n, int is_h)
     double *tH[3][2] = {{Hz, Hy}, {Hz, Hx}, {Hy, Hx}};
     double cH[3][2] = {{T->CHx * T->E0y, T->CHx * T->E0z}, {T->CHy * T->E0x, T->CHy * T->E0z}, {T->CHz * T->E0x, T->CHz * T->E0y}};
     double sH[3][2] = {{+1, -1}, {-1, +1}, {+1, -1}};
-    const int order[3] = {1, 0, 2}; /* y faces, then x, then z */
     int far = (s->flags & ORC_FIX_TFSF_FARFACE) ? 0 : 1;
-    for (int oa = 0; oa < 3; oa++) {
-        int a = order[oa];
-        for (int e = 0; e < 2; e++) {
+    int oa = a == 1 ? 0 : a == 0 ? 1 : 2; /* records: y faces, then x, then z */
+    {
+        {
             int f = oa * 2 + e;
             double flip = e ? -1.0 : 1.0;
             for (int q = T->face_start[f]; q < T->face_start[f + 1]; q++) {
@@ -1038,6 +1038,9 @@ static void tfsf_apply(orc_sim *s, int n, int is_h)
                         inc = interp(T->Hinc, (double)n - pt->dE[c]);
                         prof = pt->pE[c];
                     }
+                    /* TF_SF.cpp:1126-1143, 1375-1392: the z1 face variants
+                     * leave the profile factor out */
+                    if (variant && a == 2 && e == 0) prof = 1.0;
                     double *F = tg[a][c];
                     if (sg[a][c] * flip < 0) F[p] = F[p] - c2[c] * inc * prof;
                     else F[p] = F[p] + c2[c] * inc * prof;
@@ -1045,6 +1048,30 @@ static void tfsf_apply(orc_sim *s, int n, int is_h)
             }
         }
     }
+}
+
+/* the whole update: y faces, then x, then z (TF_SF.cpp:988-1089, 1201-1301) */
+static void tfsf_apply(orc_sim *s, int n, int is_h)
+{
+    const int order[3] = {1, 0, 2};
+    for (int oa = 0; oa < 3; oa++)
+        for (int e = 0; e < 2; e++) tfsf_face_apply(s, n, is_h, order[oa], e, 0);
+}
+
+/* TF_SF::{x,y,z}{1,2}Update{E,H}Field in the order orc_set_tfsf_faces gave */
+static void tfsf_apply_faces(orc_sim *s, int n, int is_h)
+{
+    for (int q = 0; q < s->nfaces; q++) tfsf_face_apply(s, n, is_h, s->faces[q] / 2, s->faces[q] % 2, 1);
+}
+
+int orc_set_tfsf_faces(orc_sim *s, const int *faces, int n)
+{
+    if (n < 0 || n > 64) { snprintf(g_err, sizeof g_err, "0..64 faces"); return -1; }
+    for (int q = 0; q < n; q++)
+        if (faces[q] < 0 || faces[q] > 5) { snprintf(g_err, sizeof g_err, "face %d not in 0..5", faces[q]); return -1; }
+    for (int q = 0; q < n; q++) s->faces[q] = faces[q];
+    s->nfaces = n;
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1373,10 +1400,10 @@ int orc_step(orc_sim *s, int count)
         int n = s->ndone;
         if (n >= s->nt) { snprintf(g_err, sizeof g_err, "run finished (%d steps)", s->nt); return -1; }
         basic_H(s);
-        if (s->tf.ok) tfsf_apply(s, n, 1);
+        if (s->tf.ok) { if (s->nfaces) tfsf_apply_faces(s, n, 1); else tfsf_apply(s, n, 1); }
         if (s->pml.ok) cpml_update_H(s);
         basic_E(s);
-        if (s->tf.ok) tfsf_apply(s, n, 0);
+        if (s->tf.ok) { if (s->nfaces) tfsf_apply_faces(s, n, 0); else tfsf_apply(s, n, 0); }
         for (int q = 0; q < s->nsrc; q++) {
             size_t p = IDX(s, s->src[4 * q], s->src[4 * q + 1], s->src[4 * q + 2]);
             double *Fc = s->F[s->src[4 * q + 3]];
@@ -1467,6 +1494,40 @@ void orc_set_field(orc_sim *s, int comp, const double *in) { memcpy(s->F[comp], 
 int orc_gain_ncells(const orc_sim *s) { return s->gn.ncell; }
 void orc_gain_pop(const orc_sim *s, double *out) { memcpy(out, s->gn.N, sizeof(double) * 4 * (size_t)s->gn.ncell); }
 int orc_dl_ncells(const orc_sim *s) { return s->dl.ncell; }
+int orc_dl_state(const orc_sim *s, double *out)
+{
+    const dl_t *D = &s->dl;
+    if (out && D->ok) memcpy(out, D->st, sizeof(double) * (size_t)D->ncell * 3 * (D->nD + 2 * D->nL + 1));
+    return D->ok ? D->ncell : 0;
+}
+void orc_dl_poles(const orc_sim *s, int *nD_nL)
+{
+    nD_nL[0] = s->dl.ok ? s->dl.nD : 0;
+    nD_nL[1] = s->dl.ok ? s->dl.nL : 0;
+}
+int orc_gain_state(const orc_sim *s, double *out)
+{
+    const gain_t *G = &s->gn;
+    if (out && G->ok)
+        for (int c = 0; c < G->ncell; c++)
+            for (int q = 0; q < 3; q++) {
+                out[(size_t)c * 6 + q] = G->P[(size_t)c * 3 + q];
+                out[(size_t)c * 6 + 3 + q] = G->P1[(size_t)c * 3 + q];
+            }
+    return G->ok ? G->ncell : 0;
+}
+int orc_special_cells(const orc_sim *s, int kind, int *ijk)
+{
+    int n = kind ? (s->gn.ok ? s->gn.ncell : 0) : (s->dl.ok ? s->dl.ncell : 0);
+    const int *cell = kind ? s->gn.cell : s->dl.cell;
+    for (int r = 0; ijk && r < n; r++) {
+        size_t p = (size_t)cell[r];
+        ijk[3 * r] = (int)(p / ((size_t)(s->Ny + 1) * (s->Nz + 1)));
+        ijk[3 * r + 1] = (int)((p / (size_t)(s->Nz + 1)) % (size_t)(s->Ny + 1));
+        ijk[3 * r + 2] = (int)(p % (size_t)(s->Nz + 1));
+    }
+    return n;
+}
 
 int orc_source_wave(const orc_sim *s, int q, double *out)
 {

@@ -109,6 +109,21 @@ void orc_set_field(orc_sim *s, int comp, const double *in);
 int orc_gain_ncells(const orc_sim *s);
 void orc_gain_pop(const orc_sim *s, double *out);
 int orc_dl_ncells(const orc_sim *s);
+/* TF/SF face variants: with n > 0 every later step applies the listed faces'
+ * TF_SF::{x,y,z}{1,2}UpdateHField, then after the E update their
+ * Update*EField forms, in that order, in place of UpdateHField/UpdateEField
+ * (face = 2 * axis + high: 0 x1, 1 x2, 2 y1, 3 y2, 4 z1, 5 z2; the z1 forms
+ * without the profile factor, TF_SF.cpp:1126-1143, 1375-1392).  n = 0
+ * restores the whole update. */
+int orc_set_tfsf_faces(orc_sim *s, const int *faces, int n);
+/* Drude-Lorentz state [cell][3][nD + 2 nL + 1] (JD, JL, JL_1, E_1 per
+ * component) and gain polarisation [cell][6] (P x y z, previous P x y z);
+ * both return the cell count (out may be NULL). */
+int orc_dl_state(const orc_sim *s, double *out);
+void orc_dl_poles(const orc_sim *s, int *nD_nL);
+int orc_gain_state(const orc_sim *s, double *out);
+/* i j k of each Drude-Lorentz (kind 0) or gain (kind 1) cell */
+int orc_special_cells(const orc_sim *s, int kind, int *ijk);
 /* Point source q's waveform, one value per time index 0..nt (added after the
  * E update of step n); returns nt + 1, or 0 for no such source. */
 int orc_source_wave(const orc_sim *s, int q, double *out);

@@ -70,6 +70,9 @@ def lib():
             ("orc_dl_ncells", [vp], C.c_int), ("orc_dl_coefs", [vp, dp], C.c_int),
             ("orc_source_wave", [vp, C.c_int, dp], C.c_int),
             ("orc_dl_0d", [vp, C.c_int, dp, dp], C.c_int), ("orc_gain_0d", [vp, C.c_int, dp, dp, dp], C.c_int),
+            ("orc_set_tfsf_faces", [vp, ip, C.c_int], C.c_int), ("orc_dl_state", [vp, dp], C.c_int),
+            ("orc_gain_state", [vp, dp], C.c_int), ("orc_special_cells", [vp, C.c_int, ip], C.c_int),
+            ("orc_dl_poles", [vp, ip], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -253,6 +256,37 @@ class OracleSim:
     def dl_ncells(self):
         return lib().orc_dl_ncells(self.h)
 
+    def set_tfsf_faces(self, faces):
+        """Step with TF_SF's one-face updates in this order (face = 2 * axis
+        + high: 0 x1 ... 5 z2) instead of the whole update; [] restores it."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        if lib().orc_set_tfsf_faces(self.h, _ip(f), len(f)) < 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def dl_state(self):
+        """[cell][3][nD + 2 nL + 1]: JD poles, JL poles, JL_1 poles, E_1."""
+        poles = np.zeros(2, np.int32)
+        lib().orc_dl_poles(self.h, _ip(poles))
+        per = int(poles[0]) + 2 * int(poles[1]) + 1
+        n = lib().orc_dl_state(self.h, None)
+        a = np.zeros(max(n, 1) * 3 * per, np.float64)
+        lib().orc_dl_state(self.h, _dp(a))
+        return a[:n * 3 * per].reshape(n, 3, per)
+
+    def gain_state(self):
+        """[cell][6]: P x y z, previous P x y z."""
+        n = lib().orc_gain_state(self.h, None)
+        a = np.zeros((max(n, 1), 6), np.float64)
+        lib().orc_gain_state(self.h, _dp(a))
+        return a[:n]
+
+    def special_cells(self, kind):
+        """[cell][3] i j k of the Drude-Lorentz (0) or gain (1) cells."""
+        n = lib().orc_special_cells(self.h, kind, None)
+        a = np.zeros((max(n, 1), 3), np.int32)
+        lib().orc_special_cells(self.h, kind, _ip(a))
+        return a[:n]
+
     def dl_coefs(self):
         a = np.zeros(64, np.float64)
         n = lib().orc_dl_coefs(self.h, _dp(a))
@@ -302,10 +336,16 @@ def ref_tables(case, k0=None):
     return r
 
 
-def ref_run(case, nsteps, src=None, k0=None):
-    """Run the reference loop; returns (fields[6], port samples list)."""
+FACE_NAMES = ("x1", "x2", "y1", "y2", "z1", "z2")
+
+
+def ref_run(case, nsteps, src=None, k0=None, faces=None):
+    """Run the reference loop; returns (fields[6], port samples list).
+    faces: TF/SF through the one-face methods in this order (0 x1 ... 5 z2)."""
     out = tempfile.mkdtemp(prefix="refdrv_")
     args = [REFDRV, "run", case.dir, out, str(nsteps)] + _port_arg(case)
+    if faces:
+        args += ["--faces", ",".join(FACE_NAMES[f] for f in faces)]
     if src is not None:
         args += ["--src"] + [str(x) for x in src]
     if k0 is not None:
@@ -321,6 +361,27 @@ def ref_run(case, nsteps, src=None, k0=None):
     for q in range(len(case.ports)):
         ports.append(np.fromfile(os.path.join(out, "port%d_samples.f64" % q), dtype=np.float64).reshape(nsteps, -1, 6))
     return fields, ports
+
+
+def ref_source(case, nhalf, amp, loc):
+    """Source::Pulse and Source::ContinuousWave of the case's source file at
+    ntimestep = q / 2 (q < nhalf), amplitude amp, location loc (metres)."""
+    out = tempfile.mkdtemp(prefix="refdrv_")
+    subprocess.run([REFDRV, "source", case.dir, out, str(nhalf), repr(float(amp)), repr(float(loc))], check=True,
+                   stdout=subprocess.DEVNULL)
+    r = {k: np.fromfile(os.path.join(out, k + ".f64"), dtype=np.float64) for k in ("pulse", "cw")}
+    shutil.rmtree(out, ignore_errors=True)
+    return r
+
+
+def ref_inc(case, nq):
+    """TF_SF::Einc_onPoint, ::Hinc_onPoint at retarded = q / 4 - 1 (q < nq),
+    [q][E, H]."""
+    out = tempfile.mkdtemp(prefix="refdrv_")
+    subprocess.run([REFDRV, "inc", case.dir, out, str(nq)], check=True, stdout=subprocess.DEVNULL)
+    r = np.fromfile(os.path.join(out, "inc.f64"), dtype=np.float64).reshape(nq, 2)
+    shutil.rmtree(out, ignore_errors=True)
+    return r
 
 
 def ref_port_text(case, nsteps, src=None):

@@ -15,10 +15,19 @@
 //   refdrv tables <case> <out>                        ID map and setup tables
 //   refdrv run <case> <out> <nsteps> [opts]           time loop, fields + ports
 //   refdrv bench <case> <nsteps> [opts]               time the loop, print JSON
+//   refdrv source <case> <out> <nhalf> <amp> <loc>    Source::Pulse and
+//                         ::ContinuousWave at ntimestep = q / 2, q < nhalf
+//                         (out/pulse.f64, out/cw.f64)
+//   refdrv inc <case> <out> <nq>                      TF_SF::Einc_onPoint and
+//                         ::Hinc_onPoint at retarded = q / 4 - 1, q < nq
+//                         (out/inc.f64, E and H interleaved)
 // opts: --src i j k amp   soft Ez source, amplitude * Source::Pulse(n)
 //       --k0 v            value the TF_SF object's unset k0 holds
 //       --ports f1,f2     plane port files (p:name) or sphere (s:name)
 //       --notfsf          skip TF/SF even when TF_SF.txt exists
+//       --faces f1,f2     TF/SF through the one-face methods (x1 ... z2,
+//                         x1UpdateHField ... then x1UpdateEField ...) in
+//                         this order instead of UpdateHField/UpdateEField
 //       --export          (run) also write each port's text through the
 //                         reference's own Export_* methods: port{q}_Ex.txt and
 //                         port{q}_I.txt (Export_Ex/EIntensity_onPort(ofstream&)
@@ -78,6 +87,7 @@ struct Opts {
     int notfsf = 0;
     int exports = 0;
     std::vector<std::string> ports;
+    std::vector<int> faces;  // 0 x1, 1 x2, 2 y1, 3 y2, 4 z1, 5 z2
 };
 
 static Opts parse_opts(int argc, char **argv, int first)
@@ -94,6 +104,16 @@ static Opts parse_opts(int argc, char **argv, int first)
             o.notfsf = 1;
         } else if (s == "--export") {
             o.exports = 1;
+        } else if (s == "--faces" && a + 1 < argc) {
+            std::string l = argv[++a];
+            for (size_t p0 = 0; p0 + 1 < l.size() + 1; p0 += 3) {
+                const std::string f = l.substr(p0, 2);
+                if (f.size() != 2 || f[0] < 'x' || f[0] > 'z' || (f[1] != '1' && f[1] != '2')) {
+                    std::fprintf(stderr, "bad face %s\n", f.c_str());
+                    std::exit(2);
+                }
+                o.faces.push_back(2 * (f[0] - 'x') + (f[1] - '1'));
+            }
         } else if (s == "--ports" && a + 1 < argc) {
             std::string l = argv[++a];
             size_t p0 = 0;
@@ -143,14 +163,22 @@ static Sim setup(const Opts &o)
     return m;
 }
 
+typedef void (TF_SF::*FaceFn)(int, Field &, Structure &);
+static const FaceFn kFaceH[6] = {&TF_SF::x1UpdateHField, &TF_SF::x2UpdateHField, &TF_SF::y1UpdateHField,
+                                 &TF_SF::y2UpdateHField, &TF_SF::z1UpdateHField, &TF_SF::z2UpdateHField};
+static const FaceFn kFaceE[6] = {&TF_SF::x1UpdateEField, &TF_SF::x2UpdateEField, &TF_SF::y1UpdateEField,
+                                 &TF_SF::y2UpdateEField, &TF_SF::z1UpdateEField, &TF_SF::z2UpdateEField};
+
 // main.cpp:267-343 order: H, TF/SF-H, CPML-H, E, TF/SF-E, source, CPML-E
 static void step(Sim &m, const Opts &o, int n)
 {
     BasicHFieldUpdate(*m.F, *m.S);
-    if (m.tf) m.tf->UpdateHField(n, *m.F, *m.S);
+    if (m.tf && o.faces.empty()) m.tf->UpdateHField(n, *m.F, *m.S);
+    if (m.tf) for (int f : o.faces) (m.tf->*kFaceH[f])(n, *m.F, *m.S);
     if (m.pml) m.pml->UpdateCPMLHFields(*m.F);
     BasicEFieldUpdate(*m.F, *m.S);
-    if (m.tf) m.tf->UpdateEField(n, *m.F, *m.S);
+    if (m.tf && o.faces.empty()) m.tf->UpdateEField(n, *m.F, *m.S);
+    if (m.tf) for (int f : o.faces) (m.tf->*kFaceE[f])(n, *m.F, *m.S);
     if (m.src) m.F->Ez[o.si][o.sj][o.sk] = m.F->Ez[o.si][o.sj][o.sk] + m.src->Pulse(o.samp, (double)n, 0.0);
     if (m.pml) m.pml->UpdateCPMLEFields(*m.F);
 }
@@ -325,6 +353,36 @@ static int cmd_bench(int nsteps, const Opts &o)
     return 0;
 }
 
+static int cmd_source(const std::string &out, int nhalf, double amp, double loc)
+{
+    Structure S("material.txt");
+    S.generate_mesh("structure.txt");
+    S.InitializeBasicFDTDCoefficients();
+    Source src(S.dt, (char *)"source.txt");
+    std::vector<double> pulse, cw;
+    for (int q = 0; q < nhalf; q++) {
+        pulse.push_back(src.Pulse(amp, q / 2.0, loc));
+        cw.push_back(src.ContinuousWave(amp, q / 2.0, loc));
+    }
+    wr(out + "/pulse.f64", pulse.data(), pulse.size() * 8);
+    wr(out + "/cw.f64", cw.data(), cw.size() * 8);
+    return 0;
+}
+
+static int cmd_inc(const std::string &out, int nq)
+{
+    Opts o;
+    Sim m = setup(o);
+    if (!m.tf) { std::fprintf(stderr, "no TF_SF.txt\n"); return 2; }
+    std::vector<double> v;
+    for (int q = 0; q < nq; q++) {
+        v.push_back(m.tf->Einc_onPoint(q / 4.0 - 1.0));
+        v.push_back(m.tf->Hinc_onPoint(q / 4.0 - 1.0));
+    }
+    wr(out + "/inc.f64", v.data(), v.size() * 8);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 3) {
@@ -339,6 +397,8 @@ int main(int argc, char **argv)
     if (cmd == "tables" && argc >= 4) return cmd_tables(argv[3], parse_opts(argc, argv, 4));
     if (cmd == "run" && argc >= 5) return cmd_run(argv[3], atoi(argv[4]), parse_opts(argc, argv, 5));
     if (cmd == "bench" && argc >= 4) return cmd_bench(atoi(argv[3]), parse_opts(argc, argv, 4));
+    if (cmd == "inc" && argc >= 5) return cmd_inc(argv[3], atoi(argv[4]));
+    if (cmd == "source" && argc >= 7) return cmd_source(argv[3], atoi(argv[4]), atof(argv[5]), atof(argv[6]));
     std::fprintf(stderr, "bad arguments\n");
     return 2;
 }

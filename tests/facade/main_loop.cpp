@@ -5,8 +5,19 @@
 //   main_loop --material F --structure F [--pml F] [--tfsf F] [--dl F]
 //             [--gain F] [--source F] [--plane F]... [--sphere F]...
 //             [--src i j k comp amp] [--steps N] [--flags U] [--out DIR]
-//             [--mesh-only] [--images CMAX]
+//             [--mesh-only] [--images CMAX] [--cfs] [--faces x1,z1,...]
+//             [--state]
 //
+// --cfs drives the CPML through the CFS_PML class (Initialize_PML,
+// UpdatePMLHFields, UpdatePMLEFields) instead of CPML.  --faces applies the
+// TF/SF correction through the one-face methods (x1UpdateHField ... then
+// x1UpdateEField ...) in the given order instead of UpdateHField /
+// UpdateEField.  --state, after the last step, writes DIR/dl_state.f64
+// (per Drude-Lorentz cell from Drude_Lorentz::DLM(): ix jy kz, then per
+// component JD[], JL[], JL_1[] and E_1), DIR/gain_state.f64 (per gain cell
+// from FourLevelLaser: i j k, N0..N3, Pax Pay Paz) and DIR/inc.f64
+// (TF_SF::Einc_onPoint and Hinc_onPoint at retarded = q / 4 - 1,
+// q < 4 steps + 4).
 // --mesh-only stops after Structure::generate_mesh (no device): writes
 // DIR/id.i16 (ID map, [i][j][k]) and prints Nx Ny Nz dx dy dz dt.
 // Writes DIR/field_{Ex..Hz}.f64 (final fields, [i][j][k]), per port q:
@@ -39,7 +50,8 @@ int main(int argc, char **argv)
     std::vector<double> src_amp;
     int steps = 10;
     unsigned flags = 0;
-    bool mesh_only = false;
+    bool mesh_only = false, cfs = false, state = false;
+    std::vector<int> faces;
     double images_cmax = -1;
     for (int a = 1; a < argc; a++) {
         std::string k = argv[a];
@@ -61,6 +73,18 @@ int main(int argc, char **argv)
         else if (k == "--out") out = next();
         else if (k == "--mesh-only") mesh_only = true;
         else if (k == "--images") images_cmax = std::atof(next().c_str());
+        else if (k == "--cfs") cfs = true;
+        else if (k == "--state") state = true;
+        else if (k == "--faces") {
+            const std::string l = next();
+            for (size_t p = 0; p < l.size(); p += 3) {
+                if (l.size() < p + 2 || l[p] < 'x' || l[p] > 'z' || (l[p + 1] != '1' && l[p + 1] != '2')) {
+                    std::fprintf(stderr, "bad face list %s\n", l.c_str());
+                    return 2;
+                }
+                faces.push_back(2 * (l[p] - 'x') + (l[p + 1] - '1'));
+            }
+        }
         else if (k == "--src") {
             for (int q = 0; q < 4; q++) src_ijkc.push_back(std::atoi(next().c_str()));
             src_amp.push_back(std::atof(next().c_str()));
@@ -88,7 +112,9 @@ int main(int argc, char **argv)
             S.add_point_source(src_ijkc[4 * q], src_ijkc[4 * q + 1], src_ijkc[4 * q + 2], src_ijkc[4 * q + 3], src_amp[q]);
         cpsfdtd::Field F(S, "3D");
         std::unique_ptr<cpsfdtd::CPML> cpml;
-        if (!pml.empty()) { cpml.reset(new cpsfdtd::CPML(S, pml.c_str())); cpml->InitializeCPML(S); }
+        std::unique_ptr<cpsfdtd::CFS_PML> cfspml;
+        if (!pml.empty() && cfs) { cfspml.reset(new cpsfdtd::CFS_PML(S, pml.c_str())); cfspml->Initialize_PML(S); }
+        else if (!pml.empty()) { cpml.reset(new cpsfdtd::CPML(S, pml.c_str())); cpml->InitializeCPML(S); }
         std::unique_ptr<cpsfdtd::TF_SF> tf;
         if (!tfsf.empty()) tf.reset(new cpsfdtd::TF_SF(S, tfsf.c_str()));
         std::unique_ptr<cpsfdtd::Drude_Lorentz> DL;
@@ -112,17 +138,28 @@ int main(int argc, char **argv)
         for (size_t q = 0; q < sp.size(); q++)
             sp[q]->Export_Port((out + "/port" + std::to_string(pp.size() + q) + "_points.txt").c_str());
 
+        using FaceFn = void (cpsfdtd::TF_SF::*)(int, cpsfdtd::Field &, cpsfdtd::Structure &);
+        const FaceFn faceH[6] = {&cpsfdtd::TF_SF::x1UpdateHField, &cpsfdtd::TF_SF::x2UpdateHField,
+                                 &cpsfdtd::TF_SF::y1UpdateHField, &cpsfdtd::TF_SF::y2UpdateHField,
+                                 &cpsfdtd::TF_SF::z1UpdateHField, &cpsfdtd::TF_SF::z2UpdateHField};
+        const FaceFn faceE[6] = {&cpsfdtd::TF_SF::x1UpdateEField, &cpsfdtd::TF_SF::x2UpdateEField,
+                                 &cpsfdtd::TF_SF::y1UpdateEField, &cpsfdtd::TF_SF::y2UpdateEField,
+                                 &cpsfdtd::TF_SF::z1UpdateEField, &cpsfdtd::TF_SF::z2UpdateEField};
         for (int n = 0; n < steps; ++n) {
             cpsfdtd::BasicHFieldUpdate(F, S);
-            if (tf) tf->UpdateHField(n, F, S);
+            if (tf && faces.empty()) tf->UpdateHField(n, F, S);
+            if (tf) for (int f : faces) ((*tf).*faceH[f])(n, F, S);
             if (cpml) cpml->UpdateCPMLHFields(F);
+            if (cfspml) cfspml->UpdatePMLHFields(F);
             if (laser) laser->UpdatePolarizationDensities(F);
             cpsfdtd::BasicEFieldUpdate(F, S);
             if (DL) DL->EFieldUpdate(F, S);
             if (laser) { laser->UpdateElectricField(F); laser->UpdateStatesPopulations(F); }
-            if (tf) tf->UpdateEField(n, F, S);
+            if (tf && faces.empty()) tf->UpdateEField(n, F, S);
+            if (tf) for (int f : faces) ((*tf).*faceE[f])(n, F, S);
             if (!src_amp.empty()) cpsfdtd::UpdateSources(S, n);
             if (cpml) cpml->UpdateCPMLEFields(F);
+            if (cfspml) cfspml->UpdatePMLEFields(F);
             for (size_t q = 0; q < nports; q++) {
                 cpsfdtd::detail::PortBase &P = q < pp.size() ? (cpsfdtd::detail::PortBase &)*pp[q]
                                                               : (cpsfdtd::detail::PortBase &)*sp[q - pp.size()];
@@ -160,6 +197,45 @@ int main(int argc, char **argv)
             FILE *fh = std::fopen((out + "/port" + std::to_string(q) + "_samples.f64").c_str(), "wb");
             std::fwrite(samples[q].data(), sizeof(double), samples[q].size(), fh);
             std::fclose(fh);
+        }
+        auto dump = [&](const char *file, const std::vector<double> &v) {
+            FILE *fh = std::fopen((out + "/" + file).c_str(), "wb");
+            std::fwrite(v.data(), sizeof(double), v.size(), fh);
+            std::fclose(fh);
+        };
+        if (state && DL) {
+            std::vector<double> v;
+            for (const cpsfdtd::DL_material &M : DL->DLM()) {
+                v.insert(v.end(), {(double)M.ix, (double)M.jy, (double)M.kz});
+                const std::vector<double> *JD[3] = {&M.JDx, &M.JDy, &M.JDz}, *JL[3] = {&M.JLx, &M.JLy, &M.JLz},
+                                          *JL1[3] = {&M.JLx_1, &M.JLy_1, &M.JLz_1};
+                const double E1[3] = {M.Ex_1, M.Ey_1, M.Ez_1};
+                for (int c = 0; c < 3; c++) {
+                    v.insert(v.end(), JD[c]->begin(), JD[c]->end());
+                    v.insert(v.end(), JL[c]->begin(), JL[c]->end());
+                    v.insert(v.end(), JL1[c]->begin(), JL1[c]->end());
+                    v.push_back(E1[c]);
+                }
+            }
+            dump("dl_state.f64", v);
+        }
+        if (state && laser) {
+            const std::vector<int> loc = laser->location();
+            const std::vector<double> n0 = laser->N0(), n1 = laser->N1(), n2 = laser->N2(), n3 = laser->N3();
+            const std::vector<double> px = laser->Pax(), py = laser->Pay(), pz = laser->Paz();
+            std::vector<double> v;
+            for (size_t r = 0; r < n0.size(); r++)
+                v.insert(v.end(), {(double)loc[3 * r], (double)loc[3 * r + 1], (double)loc[3 * r + 2], n0[r], n1[r], n2[r], n3[r],
+                                   px[r], py[r], pz[r]});
+            dump("gain_state.f64", v);
+        }
+        if (state && tf) {
+            std::vector<double> v;
+            for (int q = 0; q < 4 * steps + 4; q++) {
+                v.push_back(tf->Einc_onPoint(q / 4.0 - 1.0));
+                v.push_back(tf->Hinc_onPoint(q / 4.0 - 1.0));
+            }
+            dump("inc.f64", v);
         }
         static const char *names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
         for (int c = 0; c < 6; c++) {

@@ -2,7 +2,7 @@
 core (oracle/_ref/refdrv, built by `make -C oracle ref` from /root/reference).
 
 Run in a container that has /root/reference:
-    python tests/golden/make_golden.py [shipped] [mixed] [tiny] [text] [extra]
+    python tests/golden/make_golden.py [shipped] [mixed] [tiny] [text] [extra] [facade]
 (no argument: all of them)
 
 Fixtures (all data, no reference source):
@@ -23,6 +23,18 @@ Fixtures (all data, no reference source):
                         coefficient tables of tests.cases.mixed(magnetic=True)
                         and digests after 100 steps of tests.cases.tiny(
                         magnetic=True)
+  ref_facade.npz        TF/SF through the one-face methods (TF_SF.cpp:1090-1410):
+                        SHA-256 digests and L2 norms of the fields and port
+                        samples after 80 steps of tests.cases.tiny with the
+                        faces (0 x1 ... 5 z2) in three orders; Source::Pulse
+                        and Source::ContinuousWave (Sources.cpp:60-76) of
+                        tests.cases.tiny's source file at ntimestep = q / 2,
+                        q < 2000, amplitude 1.7, locations 0 and 2e-7 m;
+                        TF_SF::Einc_onPoint and ::Hinc_onPoint (TF_SF.cpp:
+                        966-986) of tests.cases.tiny at retarded = q / 4 - 1,
+                        q < 4 nt + 4: the pulse (nt = 80) and the continuous
+                        wave (nt = 300: its ramp writes past the tables when
+                        Nt is shorter, F19)
   ref_port_text.npz     the text the reference's Ports.cpp Export_* methods write
                         over 40 steps of tests.cases.tiny(text_ports=True) (a
                         line, a 2-D plane and a partial sphere port; plane
@@ -206,7 +218,33 @@ def make_extra():
     np.savez_compressed(os.path.join(HERE, "ref_extra.npz"), **out)
 
 
-MAKERS = {"shipped": make_shipped, "mixed": make_mixed, "tiny": make_tiny, "text": make_text, "extra": make_extra}
+FACE_ORDERS = ([2, 0, 4, 5, 1, 3], [4], [3, 1])
+SOURCE_LOCS = (0.0, 2e-7)
+
+
+def make_facade():
+    out = {"face_nt": np.array([80])}
+    for q, faces in enumerate(FACE_ORDERS):
+        case = cases.tiny(tempfile.mkdtemp(), nt=80)
+        F, P = po.ref_run(case, 80, src=(8, 9, 10, 2.0), faces=faces)
+        out["faces%d" % q] = np.array(faces)
+        out["faces%d_field_sha256" % q] = np.array([digest(F[c]) for c in range(6)])
+        out["faces%d_field_l2" % q] = np.array([float(np.linalg.norm(F[c])) for c in range(6)])
+        out["faces%d_port0_sha256" % q] = digest(P[0])
+    case = cases.tiny(tempfile.mkdtemp(), nt=80)
+    for q, loc in enumerate(SOURCE_LOCS):
+        r = po.ref_source(case, 2000, 1.7, loc)
+        out["pulse%d" % q] = r["pulse"]
+        out["cw%d" % q] = r["cw"]
+    out["source_locs"] = np.array(SOURCE_LOCS)
+    for cw, nt in ((0, 80), (1, 300)):
+        out["inc_cw%d" % cw] = po.ref_inc(cases.tiny(tempfile.mkdtemp(), nt=nt, cw=cw), 4 * nt + 4)
+        out["inc_cw%d_nt" % cw] = np.array([nt])
+    np.savez_compressed(os.path.join(HERE, "ref_facade.npz"), **out)
+
+
+MAKERS = {"shipped": make_shipped, "mixed": make_mixed, "tiny": make_tiny, "text": make_text, "extra": make_extra,
+          "facade": make_facade}
 
 
 def main():
