@@ -386,6 +386,63 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack
             "own streams" % pack_cells if pack_cells else "one stream per request"}
 
 
+def serving_c5_sequential(nreq=40, precision=32):
+    """The same first ``nreq`` requests solved one after another on the
+    engine alone (set-up, every step, port DFT read back, close): the rate a
+    single stream of work gets, beside the server's concurrent streams, and
+    a run whose kernel mix rocprofv3 can trace from one thread."""
+    import cpsfdtd_amd as eng
+    from cpsfdtd_amd import configs
+    root = tempfile.mkdtemp(prefix="bench_c5s_")
+    rq = configs.c5_requests(nreq)
+    work, t0 = 0, time.perf_counter()
+    for k, (_, p) in enumerate(rq):
+        d = os.path.join(root, "r%d" % k)
+        os.makedirs(d)
+        case = configs.c5(d, p)
+        e = eng.Engine(case, precision)
+        e.step(case.nt)
+        e.port_dft(0)
+        e.close()
+        work += p["N"][0] * p["N"][1] * p["N"][2] * p["steps"]
+        if k % 10 == 9:
+            log("bench: sequential, %d of %d requests (%.0f s)" % (k + 1, nreq, time.perf_counter() - t0))
+    wall = time.perf_counter() - t0
+    return {"workload": "c5: first %d of the 2000 requests, solved one after another, FP%d" % (nreq, precision),
+            "requests": nreq, "value": round(work / wall / 1e6, 2), "unit": "Mcell-updates/s", "wall_s": round(wall, 2)}
+
+
+def serving_c5_kernels(nreq=40, precision=32):
+    """Per-kernel time mix of the first ``nreq`` requests, each solved alone
+    with HIP events around every launch (cfd_kernel_times; eager launches,
+    no graphs), summed over the requests: total ms, launches, mean us and
+    share of the kernel time, and the requests' cells x steps."""
+    import cpsfdtd_amd as eng
+    from cpsfdtd_amd import configs
+    root = tempfile.mkdtemp(prefix="bench_c5k_")
+    tot, work = {}, 0
+    for k, (_, p) in enumerate(configs.c5_requests(nreq)):
+        d = os.path.join(root, "r%d" % k)
+        os.makedirs(d)
+        case = configs.c5(d, p)
+        e = eng.Engine(case, precision)
+        e.set_timing(True)
+        e.step(case.nt)
+        for name, (ms, cnt) in e.kernel_times().items():
+            a = tot.setdefault(name, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+        e.close()
+        work += p["N"][0] * p["N"][1] * p["N"][2] * p["steps"]
+    all_ms = sum(v[0] for v in tot.values())
+    return {"workload": "c5: first %d of the 2000 requests, each alone, FP%d, HIP events per launch" % (nreq, precision),
+            "cell_updates": work, "kernel_ms": round(all_ms, 2),
+            "kernel_rate_mcells": round(work / (all_ms / 1e3) / 1e6, 1),
+            "kernels": {k: {"total_ms": round(v[0], 2), "launches": v[1], "avg_us": round(v[0] / v[1] * 1e3, 2),
+                            "share": round(v[0] / all_ms, 4)}
+                        for k, v in sorted(tot.items(), key=lambda kv: -kv[1][0])}}
+
+
 def serving_c2(nreq=4, precision=32, steps=4000):
     """``nreq`` copies of the 256^3 workload (the whole 4000-step run each)
     submitted at once to the server with its defaults: served rate over the
@@ -493,7 +550,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served c5 measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the c1 / c3 measurements")
     ap.add_argument("--serve-requests", type=int, default=120)
-    ap.add_argument("--only", choices=["c1", "c3", "c4", "serve", "serve-c2"],
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "serve", "serve-seq", "serve-kernels", "serve-c2"],
                     help="only this figure (for per-workload profiles): print its JSON and exit")
     ap.add_argument("--no-quality", action="store_true", help="skip the two whole 4000-step quality runs")
     ap.add_argument("--scale", type=float, default=1.0)
@@ -529,7 +586,8 @@ def main():
         log("bench: only %s" % args.only)
         fig = {"c1": lambda: other_configs(names=("c1",)), "c3": lambda: other_configs(names=("c3",)),
                "c4": batched_c4, "serve": lambda: serving_c5(args.serve_requests),
-               "serve-c2": serving_c2}[args.only]()
+               "serve-seq": lambda: serving_c5_sequential(args.serve_requests),
+               "serve-kernels": lambda: serving_c5_kernels(args.serve_requests), "serve-c2": serving_c2}[args.only]()
         print(json.dumps({"only": args.only, "result": fig, "device": info["name"]}), flush=True)
         return
     warmup, steps, tsteps = whole_blocks(args.warmup), whole_blocks(args.steps), whole_blocks(args.timing_steps)
