@@ -15,7 +15,9 @@ import numpy as np
 from .case import Case  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libcpsfdtd.so")
+# CFD_LIB selects another build of the same library (the sanitizer build of
+# tests/run_sanitized.sh); the default is the in-tree one
+LIB_PATH = os.environ.get("CFD_LIB") or os.path.join(HERE, "lib", "libcpsfdtd.so")
 
 CFD_F32 = 32
 CFD_F64 = 64

@@ -77,6 +77,35 @@ def build_facade_driver(verbose=False):
     return FACADE_DRIVER
 
 
+SAN_LIB = os.path.join(LIBDIR, "sanitize", "libcpsfdtd.so")
+SANITIZE = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized(verbose=False):
+    """The same library with its host code (model, C ABI of the model,
+    server, image export) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer by g++, for CPU test runs; the device code is
+    the ordinary build.  Load it with gcc's libasan and libubsan preloaded
+    (tests/run_sanitized.sh)."""
+    out = os.path.dirname(SAN_LIB)
+    os.makedirs(out, exist_ok=True)
+    objs = []
+    for f in HOST_SOURCES:
+        o = os.path.join(OBJ, f + ".san.o")
+        _run(["g++"] + COMMON + SANITIZE + ["-O1", "-Wall", "-pthread", "-c", os.path.join(CSRC, f), "-o", o], verbose)
+        objs.append(o)
+    hip_o = os.path.join(OBJ, "engine.hip.o")
+    if not os.path.exists(hip_o):
+        build(verbose)
+    objs.append(hip_o)
+    _run(["g++", "-shared", "-fPIC", "-pthread"] + SANITIZE + ["-o", SAN_LIB] + objs +
+         ["-L/opt/rocm/lib", "-lamdhip64", "-lz", "-Wl,-rpath,/opt/rocm/lib"], verbose)
+    return SAN_LIB
+
+
 if __name__ == "__main__":
-    build(verbose="-q" not in sys.argv)
-    print(LIB)
+    if "--sanitize" in sys.argv:
+        print(build_sanitized(verbose="-q" not in sys.argv))
+    else:
+        build(verbose="-q" not in sys.argv)
+        print(LIB)

@@ -16,7 +16,8 @@ import tempfile
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# ORC_LIB selects another build (the sanitizer build, _san/liboracle.so)
+LIB_PATH = os.environ.get("ORC_LIB") or os.path.join(HERE, "liboracle.so")
 REFDRV = os.path.join(HERE, "_ref", "refdrv")
 REFDRV_O0 = os.path.join(HERE, "_ref", "refdrv_O0")  # reference sources at -O0 (timing only)
 
