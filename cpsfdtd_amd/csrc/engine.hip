@@ -201,6 +201,11 @@ public:
     TfTarget *tfE = nullptr, *tfH = nullptr;
     int ntfE = 0, ntfH = 0, tablen = 0;
     double *tcoef = nullptr, *tdelay = nullptr, *tprof = nullptr, *einc = nullptr, *hinc = nullptr;
+    // FP32: the same lists in compact form (tf32), used by every fused path
+    bool tf32 = false;
+    TfTarget32 *tfE32 = nullptr, *tfH32 = nullptr;
+    TfTerm32 *tterm32 = nullptr;
+    float *einc32 = nullptr, *hinc32 = nullptr;
     // one face alone, uploaded at its first use: targets [face][h], terms
     TfTarget *tfFaceT[6][2] = {};
     int ntfFace[6][2] = {};
@@ -260,6 +265,29 @@ public:
         if (!tmp.empty()) HIP_OK(hipMemcpyAsync(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
         return p;
+    }
+    // the FP32 compact lists (TfTarget32 / TfTerm32) from the exact ones, when
+    // every index fits their fields
+    void compact_tfsf(const std::vector<TfTarget> &te, const std::vector<TfTarget> &th, const std::vector<double> &c,
+                      const std::vector<double> &dl, const std::vector<double> &pr) {
+        if ((long long)d.slab * (m.Nz + 1) >= (1LL << 32) || c.size() >= (1u << 27)) return;
+        std::vector<TfTarget32> e32, h32;
+        for (const std::vector<TfTarget> *tv : {&te, &th})
+            for (const TfTarget &t : *tv) {
+                if (t.count > 3 || t.comp < 0 || t.comp > 5) return;
+                (tv == &te ? e32 : h32).push_back({(unsigned)t.cell, (unsigned)t.comp << 29 | (unsigned)t.count << 27 | (unsigned)t.first});
+            }
+        std::vector<TfTerm32> terms;
+        for (size_t q = 0; q < c.size(); q++) {
+            const double base = std::ceil(dl[q]);
+            terms.push_back({(float)(c[q] * pr[q]), (int)base, (float)(base - dl[q])});
+        }
+        tfE32 = upload<TfTarget32>(e32);
+        tfH32 = upload<TfTarget32>(h32);
+        tterm32 = upload<TfTerm32>(terms);
+        einc32 = upload<float>(m.Einc);
+        hinc32 = upload<float>(m.Hinc);
+        tf32 = true;
     }
     long long dev_index(int i, int j, int k) const {
         return (long long)i + (long long)d.px * ((long long)j + (long long)d.py * k);
@@ -426,6 +454,7 @@ public:
             tablen = (int)m.Einc.size();
             sig_add(tf_sig, te); sig_add(tf_sig, th); sig_add(tf_sig, c); sig_add(tf_sig, dl); sig_add(tf_sig, pr);
             sig_add(tf_sig, m.Einc); sig_add(tf_sig, m.Hinc);
+            if constexpr (std::is_same<R, float>::value) compact_tfsf(te, th, c, dl, pr);
         }
         // sources
         if (!m.src.empty()) {
@@ -719,6 +748,13 @@ public:
         const int nt = h ? ntfH : ntfE;
         if (!nt) return;
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
+            if constexpr (std::is_same<R, float>::value) {
+                if (tf32) {
+                    hipLaunchKernelGGL((k_tfsf32<false>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr,
+                                       h ? tfH32 : tfE32, nt, tterm32, h ? einc32 : hinc32, tablen, nbase_cur, n);
+                    return;
+                }
+            }
             hipLaunchKernelGGL((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, h ? tfH : tfE, nt,
                                tcoef, tdelay, tprof, h ? einc : hinc, tablen, nbase_cur, n);
         });
@@ -1148,6 +1184,14 @@ public:
         const int nt = h ? a.ntfH : a.ntfE;
         if (!nt) return;
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
+            if constexpr (std::is_same<R, float>::value) {
+                if (a.tf32) {
+                    hipLaunchKernelGGL((k_tfsf32<true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream,
+                                       a.d, devs, h ? a.tfH32 : a.tfE32, nt, a.tterm32, h ? a.einc32 : a.hinc32, a.tablen,
+                                       nullptr, n);
+                    return;
+                }
+            }
             hipLaunchKernelGGL((k_tfsf<R, true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
                                h ? a.tfH : a.tfE, nt, a.tcoef, a.tdelay, a.tprof, h ? a.einc : a.hinc, a.tablen, nullptr, n);
         });
@@ -1306,6 +1350,8 @@ public:
             P.ntfE = S.ntfE; P.ntfH = S.ntfH; P.tablen = S.tablen;
             P.tfE = S.tfE; P.tfH = S.tfH;
             P.tcoef = S.tcoef; P.tdelay = S.tdelay; P.tprof = S.tprof; P.einc = S.einc; P.hinc = S.hinc;
+            P.tf32 = S.tf32 ? 1 : 0;
+            P.tfE32 = S.tfE32; P.tfH32 = S.tfH32; P.tterm32 = S.tterm32; P.einc32 = S.einc32; P.hinc32 = S.hinc32;
             P.special = S.special; P.nspecial = S.nspecial;
             P.nsgroup = S.nsgroup; P.sgroup = S.sgroup; P.sidx = S.sidx; P.swave = S.swave;
             P.ports = S.port_tab; P.nports = (int)S.dports.size();

@@ -928,6 +928,50 @@ __device__ __forceinline__ void tfsf_apply(const Dev<R> &d, const TfTarget &g, c
     }
     F[g.cell] = v;
 }
+// The FP32 engine's compact form of the same lists: an 8-byte target and
+// 12-byte terms (against 24 + 24 bytes), the incident tables in FP32.  A
+// term's delay d is split once on the host into the table index of its
+// retarded time at step 0 and an interpolation weight, so that the index and
+// weight at step n are exact (n - base, w) whatever the magnitude of n:
+// retarded = n - d; floor = n - base with base = ceil(d); the weight of entry
+// floor + 1 is w = base - d (0 when d is whole); no incident wave while
+// retarded <= 0, i.e. floor < 0, or floor == 0 with w == 0.
+struct TfTarget32 {
+    unsigned cell;  // device index (grids below 2^32 cells)
+    unsigned meta;  // component (0..5) << 29 | term count (0..3) << 27 | first term
+};
+struct TfTerm32 {
+    float cp;  // signed C * amplitude * transverse profile
+    int base;
+    float w;
+};
+__device__ __forceinline__ void tfsf_apply32(const Dev<float> &d, const TfTarget32 g, const TfTerm32 *terms,
+                                             const float *tab, int tablen, int n)
+{
+    float *F = d.F[g.meta >> 29];
+    float v = F[g.cell];
+    const int first = (int)(g.meta & ((1u << 27) - 1)), cnt = (int)((g.meta >> 27) & 3u);
+    for (int q = first; q < first + cnt; q++) {
+        const TfTerm32 t = terms[q];
+        int nt = n - t.base;
+        if (nt < 0 || (nt == 0 && t.w == 0.0f)) continue;
+        if (nt > tablen - 2) nt = tablen - 2;
+        v = v + t.cp * ((1.0f - t.w) * tab[nt] + t.w * tab[nt + 1]);
+    }
+    F[g.cell] = v;
+}
+template <bool B = false>
+__global__ __launch_bounds__(256) void k_tfsf32(Dev<float> d0, const Dev<float> *devs, const TfTarget32 *tg, int ntg,
+                                                const TfTerm32 *terms, const float *tab, int tablen, const int *nbase,
+                                                int n)
+{
+    const Dev<float> &d = instance<float, B>(d0, devs);
+    if (nbase) n += *nbase;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntg) return;
+    tfsf_apply32(d, tg[t], terms, tab, tablen, n);
+}
+
 template <class R, bool B = false>
 __global__ __launch_bounds__(256) void k_tfsf(Dev<R> d0, const Dev<R> *devs, const TfTarget *tg, int ntg,
                                               const double *coef, const double *delay, const double *prof,
@@ -1123,6 +1167,10 @@ struct PackMember {
     int ntfE, ntfH, tablen;         // TF/SF targets per half-step, waveform length
     const TfTarget *tfE, *tfH;
     const double *tcoef, *tdelay, *tprof, *einc, *hinc;
+    int tf32;                       // FP32 members: the compact lists below
+    const TfTarget32 *tfE32, *tfH32;
+    const TfTerm32 *tterm32;
+    const float *einc32, *hinc32;
     const long long *special;       // Drude-Lorentz / gain cells (kSpecialPml flag)
     int nspecial;
     int nsgroup;                    // point-source groups
@@ -1188,6 +1236,12 @@ __global__ __launch_bounds__(256) void k_pack_tfsf(const PackMember<R> *pm, cons
     if (n >= P.nt) return;
     const int t = (blockIdx.x - start[s]) * 256 + threadIdx.x;
     if (t >= (h ? P.ntfH : P.ntfE)) return;
+    if constexpr (std::is_same<R, float>::value) {
+        if (P.tf32) {
+            tfsf_apply32(P.d, (h ? P.tfH32 : P.tfE32)[t], P.tterm32, h ? P.einc32 : P.hinc32, P.tablen, n);
+            return;
+        }
+    }
     tfsf_apply(P.d, (h ? P.tfH : P.tfE)[t], P.tcoef, P.tdelay, P.tprof, h ? P.einc : P.hinc, P.tablen, n);
 }
 
