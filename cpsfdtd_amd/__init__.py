@@ -120,6 +120,7 @@ SIGNATURES = {
     "cfd_port_flux": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "cfd_dft_flux": ([_dp, C.c_int, C.c_int, C.c_int, C.c_double, _dp], C.c_int),
     "cfd_tfsf_tables": ([C.c_char_p, C.c_double, C.c_int, _dp, _dp], C.c_int),
+    "cfd_model_device_bytes": ([_vp, C.c_int], C.c_size_t),
     "cfd_image_colormap": ([_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_materials": ([C.POINTER(C.c_short), C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
     "cfd_image_write_png": ([C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte)], C.c_int),
@@ -688,6 +689,10 @@ class Model:
         a = np.empty(self.shape, np.int16)
         library().cfd_model_id(self.h, a.ctypes.data_as(C.POINTER(C.c_int16)))
         return a
+
+    def device_bytes(self, precision=CFD_F64):
+        """Device bytes a simulation of this model allocates (cfd_model_device_bytes)."""
+        return int(library().cfd_model_device_bytes(self.h, precision))
 
     def table(self, name):
         L = library()

@@ -1630,6 +1630,50 @@ int cfd_create_from_model(cfd_model *model, int precision, cfd_sim **out)
     });
 }
 
+size_t cfd_model_device_bytes(cfd_model *p, int precision)
+{
+    if (!p || (precision != CFD_F32 && precision != CFD_F64)) return 0;
+    const cfd::Model &m = p->m;
+    const size_t S = precision == CFD_F32 ? 4 : 8;
+    const size_t px = ((size_t)m.Nx + 1 + 31) / 32 * 32, py = (size_t)m.Ny + 1, slab = px * py;
+    const size_t ncell = slab * ((size_t)m.Nz + 1);
+    size_t b = ncell * (6 * S + 1);                      // fields, material IDs
+    b += 4 * (size_t)m.NoMedia * S;                      // aE bE aH bH
+    for (int t = 0; t < cfd::kNumStretch; t++) b += m.stretch[t].size() * S;
+    const size_t N[3] = {(size_t)m.Nx, (size_t)m.Ny, (size_t)m.Nz};
+    for (int a = 0; a < 3; a++)
+        for (int e2 = 0; e2 < 2; e2++) {
+            const size_t n = (size_t)m.npml[a][e2];
+            if (!n) continue;
+            b += (m.pbE[a][e2].size() + m.pcE[a][e2].size() + m.pbH[a][e2].size() + m.pcH[a][e2].size()) * S;
+            const size_t box = a == 0 ? n * py * (N[2] + 1) : a == 1 ? px * n * (N[2] + 1) : slab * n;
+            b += 4 * box * S;                                // psi, E and H, two per slab
+        }
+    const size_t ndl = m.dl_cells.size(), ng = m.gain_cells.size();
+    if (m.has_dl || m.has_gain) b += ncell * sizeof(int) + (ndl + ng) * sizeof(long long);  // slot map, cell list
+    if (m.has_dl) b += ((size_t)m.nD + 2 * m.nL + 1) * 3 * std::max<size_t>(ndl, 1) * S;
+    if (m.has_gain) b += 9 * std::max<size_t>(ng, 1) * S + 4 * std::max<size_t>(ng, 1) * sizeof(double);
+    if (m.has_tfsf) {
+        const size_t targets = m.tfE.size() + m.tfH.size(), terms = m.tf_terms.size(), tab = m.Einc.size();
+        b += targets * sizeof(cfd::TfTarget) + terms * 3 * sizeof(double) + 2 * tab * sizeof(double);
+        if (precision == CFD_F32) b += targets * sizeof(cfd::TfTarget32) + terms * sizeof(cfd::TfTerm32) + 2 * tab * sizeof(float);
+    }
+    const size_t nsrc = m.src.size() / 4;
+    b += nsrc * (sizeof(cfd::SourceGroup) + sizeof(int)) + m.src_wave.size() * sizeof(double);
+    b += (m.twE.size() + m.twH.size()) * sizeof(double);
+    size_t gather = 0;
+    for (const cfd::Port &P : m.ports) {
+        const size_t np = (size_t)P.npts(), nblk = (np + 255) / 256, nf = m.freqs.size();
+        b += np * sizeof(long long) + (size_t)m.nt * std::max<size_t>(nblk, 1) * sizeof(double);
+        if (m.record) b += (size_t)m.nt * np * 6 * S;
+        if (nf) b += (size_t)cfd::Sim<float>::kDftBatch * 6 * np * S + np * nf * 12 * sizeof(double);
+        gather = std::max(gather, np);
+    }
+    b += gather * 6 * S;
+    // tile tables, pass boxes, counters: small; allowance for them
+    return b + b / 64 + ((size_t)1 << 20);
+}
+
 int cfd_destroy(cfd_sim *s)
 {
     if (!s) return 0;

@@ -119,15 +119,6 @@ struct cfd_server {
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-    // Device bytes a request will need, from its cell count (fields, ID map,
-    // auxiliary slot map, CPML psi and tables generously); the simulation's
-    // own figure replaces it once it exists.
-    static size_t estimate(const cfd_config &c, const int *n) {
-        const size_t S = c.precision == CFD_F32 ? 4 : 8;
-        const size_t px = ((size_t)n[0] + 1 + 31) / 32 * 32;
-        const size_t cells = px * (n[1] + 1) * (n[2] + 1);
-        return cells * (6 * S + 1 + (c.dl || c.gain ? 4 : 0)) + cells * S;
-    }
 
     // host thread: results to host memory, device memory released
     void finish(Request &r, bool ok, std::string err) {
@@ -178,7 +169,10 @@ struct cfd_server {
         }
         int n[4] = {0, 0, 0, 0};
         cfd_model_dims(r.model, n, nullptr);
-        r.estimate = estimate(r.cfg->cfg, n);
+        // device bytes from the built model (ports, DFT, recorded samples,
+        // CPML, dispersive state included); the simulation's own figure
+        // replaces it once it exists
+        r.estimate = cfd_model_device_bytes(r.model, r.cfg->cfg.precision);
         r.cells = (double)n[0] * n[1] * n[2];
         r.work = r.cells * std::max(n[3], 1);
         return true;

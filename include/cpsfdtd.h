@@ -276,6 +276,12 @@ int cfd_model_port(cfd_model *m, int port, int *ijk);               /* returns p
 int cfd_tfsf_tables(const char *tfsf, double dt, int nt, double *einc, double *hinc);
 /* DL / gain cell lists in reference [i][j][k] flat order; returns count */
 int cfd_model_cells(cfd_model *m, int gain, long long *flat);
+/* Device bytes a simulation of this model will allocate at the given
+ * precision (fields, IDs, tables, CPML psi, dispersive and gain state, TF/SF
+ * lists, sources, port buffers, DFT accumulators, recorded samples), with a
+ * small allowance; 0 for bad arguments.  No reference counterpart: the
+ * serving loop's admission estimate. */
+size_t cfd_model_device_bytes(cfd_model *m, int precision);
 
 /* Image export of a plane port's maps, host only (OutImage.cpp; the
  * reference draws with CImg and saves with CImg::save_png).  Maps are

@@ -204,3 +204,27 @@ def test_c5_first_requests_real_size(tmp_path):
         e.close()
         srv.release(rid)
     srv.close()
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_admission_estimate_covers_the_simulation(tmp_path, precision):
+    """The server admits by cfd_model_device_bytes of the built model: for
+    cases with recorded port samples, a 64-frequency DFT, CPML on every face,
+    Drude-Lorentz and gain cells, and a c5 request, it is at least what the
+    simulation then allocates (cfd_device_bytes) and at most 10 % + 2 MB
+    above it."""
+    import cases
+    from cpsfdtd_amd.case import FIXED
+    made = [
+        cases.mixed(str(tmp_path / "a"), dl=True, gain=True, flags=FIXED),
+        cases.mixed(str(tmp_path / "b"), freqs=tuple(4e14 + 1e12 * q for q in range(64)), stride=4, record=False),
+        configs.c1(str(tmp_path / "c"), nt=300, record=True),
+        make(tmp_path, 9, SMALL[2][0]),
+    ]
+    for case in made:
+        est = eng.Model(case).device_bytes(precision)
+        e = eng.Engine(case, precision)
+        got = e.device_bytes()
+        e.close()
+        print("estimate %d, allocated %d" % (est, got), flush=True)
+        assert got <= est <= got * 1.10 + (2 << 20), (case.dir, est, got)
