@@ -523,7 +523,10 @@ public:
     // columns that hold the x CPML slabs, 64 x 4 tiles between them; one
     // plane's tile origins, x fastest within each of the two kinds.
     // CFD_TILE_EDGE32=0 gives 64-wide tiles everywhere (A/B runs).
-    std::vector<unsigned> mgrid_tiles() const {
+    // y_org: the first tile row (the E pass starts at its first updated row,
+    // j = 1 without y periodicity: row 0 would idle a quarter of the first
+    // row of tiles; measured 142.5 -> 139.6 us for k_full_e on 256^3)
+    std::vector<unsigned> mgrid_tiles(int y_org = 0) const {
         const char *v = std::getenv("CFD_TILE_EDGE32");
         const bool edge32 = !v || std::string(v) != "0";
         const int n0 = edge32 ? m.npml[0][0] : 0, n1 = edge32 ? m.npml[0][1] : 0;
@@ -533,9 +536,9 @@ public:
         const int xb = xa + 64 * nw;
         if (m.Nx > 0x7fff || m.Ny > 0xffff) throw Error("grid too large for the merged tile table");
         std::vector<unsigned> t;
-        for (int y0 = 0; y0 < m.Ny; y0 += 4)
+        for (int y0 = y_org; y0 < m.Ny; y0 += 4)
             for (int b = 0; b < nw; b++) t.push_back((unsigned)(xa + 64 * b) | (unsigned)y0 << 15);
-        for (int y0 = 0; y0 < m.Ny; y0 += 8) {
+        for (int y0 = y_org; y0 < m.Ny; y0 += 8) {
             for (int x0 = 0; x0 < xa; x0 += 32) t.push_back((unsigned)x0 | (unsigned)y0 << 15 | 1u << 31);
             for (int x0 = xb; x0 < m.Nx; x0 += 32) t.push_back((unsigned)x0 | (unsigned)y0 << 15 | 1u << 31);
         }
@@ -640,10 +643,9 @@ public:
         leanH_cs = lean_constant(leanH, false, leanH_st);
         leanE_cs = lean_constant(leanE, true, leanE_st);
         {
-            const std::vector<unsigned> t = mgrid_tiles();
-            const unsigned *td = upload<unsigned>(t);
-            mgH = MGrid{td, (int)t.size(), 0, m.Nz};
-            mgE = MGrid{td, (int)t.size(), 1, std::max(0, m.Nz - 1)};
+            const std::vector<unsigned> t = mgrid_tiles(0), te = mgrid_tiles(m.pby ? 0 : 1);
+            mgH = MGrid{upload<unsigned>(t), (int)t.size(), 0, m.Nz};
+            mgE = MGrid{upload<unsigned>(te), (int)te.size(), 1, std::max(0, m.Nz - 1)};
         }
         // every Drude-Lorentz / gain cell, in device order; cells outside the
         // interior box carry the CPML flag
