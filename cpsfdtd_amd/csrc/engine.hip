@@ -1,5 +1,6 @@
 // engine.hip -- device state of one simulation, the step sequence, and the C
 // ABI declared in include/cpsfdtd.h.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +34,22 @@ static thread_local std::string g_error;
 // ----------------------------------------------------------------------------
 // per-kernel timing with HIP events on the simulation's stream
 // ----------------------------------------------------------------------------
+// While the kernel timer brackets a launch, the first kernel it launches
+// records the timer's start and stop events in its own dispatch
+// (hipExtLaunchKernelGGL), so the time is the kernel's execution alone, not
+// the queue's event packets around it; every kernel goes through CFD_LAUNCH.
+struct LaunchEvents {
+    hipEvent_t a = nullptr, b = nullptr;
+    int n = 0;  // kernels launched while bracketed
+};
+inline thread_local LaunchEvents g_launch_ev;
+#define CFD_LAUNCH(K, G, BL, SH, ST, ...)                                                     \
+    do {                                                                                     \
+        ::cfd::LaunchEvents &ev_ = ::cfd::g_launch_ev;                                       \
+        if (ev_.a && ev_.n++ == 0) hipExtLaunchKernelGGL(K, G, BL, SH, ST, ev_.a, ev_.b, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(K, G, BL, SH, ST, __VA_ARGS__);                              \
+    } while (0)
+
 struct KernelTimer {
     bool on = false;
     struct Pending { std::string name; hipEvent_t a, b; };
@@ -51,8 +68,19 @@ struct KernelTimer {
         if (!on) { launch(); return; }
         Pending p{name, get(), get()};
         HIP_OK(hipEventRecord(p.a, s));
-        launch();
-        HIP_OK(hipEventRecord(p.b, s));
+        const LaunchEvents outer = g_launch_ev;
+        g_launch_ev = LaunchEvents{p.a, p.b, 0};
+        try {
+            launch();
+        } catch (...) {
+            g_launch_ev = outer;
+            throw;
+        }
+        const int n = g_launch_ev.n;
+        g_launch_ev = outer;
+        // one kernel: its dispatch recorded both events; several: from the
+        // first one's start to after the last
+        if (n != 1) HIP_OK(hipEventRecord(p.b, s));
         pending.push_back(p);
     }
     void collect() {
@@ -112,7 +140,7 @@ void launch_full(hipStream_t s, bool h, bool cs, bool nt, unsigned ninst, const 
                  const MGrid &g, const Box &L, const LeanStretch<R> &st)
 {
     const dim3 grid((unsigned)g.per_plane, (unsigned)g.planes, ninst), blk(256);
-#define CFD_FULL(K, CS, NT) hipLaunchKernelGGL((K<R, CS, NT, B>), grid, blk, 0, s, d0, devs, g, L, st)
+#define CFD_FULL(K, CS, NT) CFD_LAUNCH((K<R, CS, NT, B>), grid, blk, 0, s, d0, devs, g, L, st)
     if (h) {
         if (cs) { if (nt) CFD_FULL(k_full_h, true, true); else CFD_FULL(k_full_h, true, false); }
         else { if (nt) CFD_FULL(k_full_h, false, true); else CFD_FULL(k_full_h, false, false); }
@@ -132,7 +160,7 @@ template <class R, bool B>
 void launch_special(hipStream_t s, dim3 grid, const Dev<R> &d0, const Dev<R> *devs, const long long *cells, int n,
                     const SpecialList *lists, const int (&poles)[2])
 {
-#define CFD_SPECIAL(ND, NL) hipLaunchKernelGGL((k_special_e<R, B, ND, NL>), grid, dim3(256), 0, s, d0, devs, cells, n, lists)
+#define CFD_SPECIAL(ND, NL) CFD_LAUNCH((k_special_e<R, B, ND, NL>), grid, dim3(256), 0, s, d0, devs, cells, n, lists)
     if (poles[0] == 1 && poles[1] == 2) CFD_SPECIAL(1, 2);
     else if (poles[0] == 1 && poles[1] == 0) CFD_SPECIAL(1, 0);
     else if (poles[0] == 0 && poles[1] == 0) CFD_SPECIAL(0, 0);
@@ -702,8 +730,8 @@ public:
             const int skip5 = m.has_gain ? 1 : 0;
             timer.run(stream, h ? "lean_h" : "lean_e", [&] {
                 const dim3 grid((unsigned)nl), blk(TX, TY);
-#define CFD_LEAN_H(CS, NT) hipLaunchKernelGGL((k_lean_h<R, CS, NT>), grid, blk, 0, stream, d, L, leanH_st)
-#define CFD_LEAN_E(CS, NT) hipLaunchKernelGGL((k_lean_e<R, CS, NT>), grid, blk, 0, stream, d, L, leanE_st, skip5)
+#define CFD_LEAN_H(CS, NT) CFD_LAUNCH((k_lean_h<R, CS, NT>), grid, blk, 0, stream, d, L, leanH_st)
+#define CFD_LEAN_E(CS, NT) CFD_LAUNCH((k_lean_e<R, CS, NT>), grid, blk, 0, stream, d, L, leanE_st, skip5)
                 if (h) {
                     if (leanH_cs) { if (nt_fields) CFD_LEAN_H(true, true); else CFD_LEAN_H(true, false); }
                     else { if (nt_fields) CFD_LEAN_H(false, true); else CFD_LEAN_H(false, false); }
@@ -719,10 +747,10 @@ public:
         const int nb = h ? shellH_blocks : shellE_blocks;
         if (nb) {
             timer.run(stream, h ? "shell_h" : "shell_e", [&] {
-                if (h && nt_fields) hipLaunchKernelGGL((k_shell_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
-                else if (h) hipLaunchKernelGGL((k_shell_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
-                else if (nt_fields) hipLaunchKernelGGL((k_shell_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
-                else hipLaunchKernelGGL((k_shell_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
+                if (h && nt_fields) CFD_LAUNCH((k_shell_h<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
+                else if (h) CFD_LAUNCH((k_shell_h<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, shellH_set);
+                else if (nt_fields) CFD_LAUNCH((k_shell_e<R, true>), dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
+                else CFD_LAUNCH((k_shell_e<R, false>), dim3(nb), dim3(TX, TY), 0, stream, d, shellE_set);
             });
             HIP_OK(hipGetLastError());
         }
@@ -737,12 +765,12 @@ public:
 
     void launch_h(unsigned parts, const char *name) {
         const TileGrid g = tiles();
-        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_h<R>, dim3(g.gx * g.gy * g.gz), dim3(TX, TY), 0, stream, d, parts, g); });
+        timer.run(stream, name, [&] { CFD_LAUNCH(k_update_h<R>, dim3(g.gx * g.gy * g.gz), dim3(TX, TY), 0, stream, d, parts, g); });
         HIP_OK(hipGetLastError());
     }
     void launch_e(unsigned parts, const char *name) {
         const TileGrid g = tiles();
-        timer.run(stream, name, [&] { hipLaunchKernelGGL(k_update_e<R>, dim3(g.gx * g.gy * g.gz), dim3(TX, TY), 0, stream, d, parts, g); });
+        timer.run(stream, name, [&] { CFD_LAUNCH(k_update_e<R>, dim3(g.gx * g.gy * g.gz), dim3(TX, TY), 0, stream, d, parts, g); });
         HIP_OK(hipGetLastError());
     }
     void tfsf(bool h, int n) override {
@@ -752,12 +780,12 @@ public:
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
             if constexpr (std::is_same<R, float>::value) {
                 if (tf32) {
-                    hipLaunchKernelGGL((k_tfsf32<false>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr,
+                    CFD_LAUNCH((k_tfsf32<false>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr,
                                        h ? tfH32 : tfE32, nt, tterm32, h ? einc32 : hinc32, tablen, nbase_cur, n);
                     return;
                 }
             }
-            hipLaunchKernelGGL((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, h ? tfH : tfE, nt,
+            CFD_LAUNCH((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, h ? tfH : tfE, nt,
                                tcoef, tdelay, tprof, h ? einc : hinc, tablen, nbase_cur, n);
         });
         HIP_OK(hipGetLastError());
@@ -787,7 +815,7 @@ public:
         const int nt = ntfFace[face][h];
         if (!nt) return;
         timer.run(stream, h ? "tfsf_face_h" : "tfsf_face_e", [&] {
-            hipLaunchKernelGGL((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, tfFaceT[face][h], nt,
+            CFD_LAUNCH((k_tfsf<R>), dim3((nt + 255) / 256), dim3(256), 0, stream, d, nullptr, tfFaceT[face][h], nt,
                                fcoef, fdelay, fprof, h ? einc : hinc, tablen, nullptr, n);
         });
         HIP_OK(hipGetLastError());
@@ -796,7 +824,7 @@ public:
         if (!nsgroup) return;
         if (!nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
         timer.run(stream, "sources", [&] {
-            hipLaunchKernelGGL(k_sources<R>, dim3((nsgroup + 63) / 64), dim3(64), 0, stream, d, sgroup, nsgroup, sidx,
+            CFD_LAUNCH(k_sources<R>, dim3((nsgroup + 63) / 64), dim3(64), 0, stream, d, sgroup, nsgroup, sidx,
                                swave, m.nt + 1, nbase_cur, n);
         });
         HIP_OK(hipGetLastError());
@@ -810,7 +838,7 @@ public:
             slot = dft_pending++;
         }
         timer.run(stream, "ports", [&] {
-            hipLaunchKernelGGL((k_port<R>), dim3(port_blocks), dim3(256), 0, stream, d, nullptr, port_tab, (int)dports.size(),
+            CFD_LAUNCH((k_port<R>), dim3(port_blocks), dim3(256), 0, stream, d, nullptr, port_tab, (int)dports.size(),
                                nbase_cur, n, slot);
         });
         HIP_OK(hipGetLastError());
@@ -821,7 +849,7 @@ public:
         const int nb = dft_pending;
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
-            hipLaunchKernelGGL((k_dft<R>), dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
+            CFD_LAUNCH((k_dft<R>), dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
                                nbase_cur, dft_first, m.dft_stride, twE, twH, (int)m.freqs.size(), m.nt);
         });
         HIP_OK(hipGetLastError());
@@ -888,7 +916,7 @@ public:
             const int G = graph_block();
             if (G && count >= G && n % G == 0 && n + G <= m.nt && dft_pending == 0 && !timer.on) {
                 if (!gexec) capture_block(G);
-                hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, stream, nbase_dev, n);
+                CFD_LAUNCH(k_set_int, dim3(1), dim3(64), 0, stream, nbase_dev, n);
                 HIP_OK(hipGetLastError());
                 HIP_OK(hipGraphLaunch(gexec, stream));
                 done += G;
@@ -946,8 +974,8 @@ public:
     }
     long long nonfinite() override {
         const long long n = d.slab * (m.Nz + 1);
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, stream, count_dev, 0);
-        hipLaunchKernelGGL(k_nonfinite<R>, dim3(1024), dim3(256), 0, stream, d, n, count_dev);
+        CFD_LAUNCH(k_set_int, dim3(1), dim3(64), 0, stream, count_dev, 0);
+        CFD_LAUNCH(k_nonfinite<R>, dim3(1024), dim3(256), 0, stream, d, n, count_dev);
         HIP_OK(hipGetLastError());
         int h = 0;
         HIP_OK(hipMemcpyAsync(&h, count_dev, sizeof h, hipMemcpyDeviceToHost, stream));
@@ -957,7 +985,7 @@ public:
     void port_gather(int q, double *host) override {
         const PortDesc<R> &p = dports.at(q);
         if (!p.npts) return;
-        hipLaunchKernelGGL(k_gather<R>, dim3((p.npts + 255) / 256), dim3(256), 0, stream, d, p.cell, p.npts, gather_buf);
+        CFD_LAUNCH(k_gather<R>, dim3((p.npts + 255) / 256), dim3(256), 0, stream, d, p.cell, p.npts, gather_buf);
         HIP_OK(hipGetLastError());
         std::vector<R> tmp((size_t)p.npts * 6);
         HIP_OK(hipMemcpyAsync(tmp.data(), gather_buf, tmp.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
@@ -967,7 +995,7 @@ public:
     void port_gather_via(int q, double *out, double *host) override {
         const PortDesc<R> &p = dports.at(q);
         if (!p.npts) return;
-        hipLaunchKernelGGL((k_gather<R, double>), dim3((p.npts + 255) / 256), dim3(256), 0, stream, d, p.cell, p.npts, out);
+        CFD_LAUNCH((k_gather<R, double>), dim3((p.npts + 255) / 256), dim3(256), 0, stream, d, p.cell, p.npts, out);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(host, out, (size_t)p.npts * 6 * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
@@ -1188,13 +1216,13 @@ public:
         timer.run(stream, h ? "tfsf_h" : "tfsf_e", [&] {
             if constexpr (std::is_same<R, float>::value) {
                 if (a.tf32) {
-                    hipLaunchKernelGGL((k_tfsf32<true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream,
+                    CFD_LAUNCH((k_tfsf32<true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream,
                                        a.d, devs, h ? a.tfH32 : a.tfE32, nt, a.tterm32, h ? a.einc32 : a.hinc32, a.tablen,
                                        nullptr, n);
                     return;
                 }
             }
-            hipLaunchKernelGGL((k_tfsf<R, true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
+            CFD_LAUNCH((k_tfsf<R, true>), dim3((nt + 255) / 256, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
                                h ? a.tfH : a.tfE, nt, a.tcoef, a.tdelay, a.tprof, h ? a.einc : a.hinc, a.tablen, nullptr, n);
         });
         HIP_OK(hipGetLastError());
@@ -1208,7 +1236,7 @@ public:
             slot = dft_pending++;
         }
         timer.run(stream, "ports", [&] {
-            hipLaunchKernelGGL((k_port<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
+            CFD_LAUNCH((k_port<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, a.d, devs,
                                ports_all, nports, nullptr, n, slot);
         });
         HIP_OK(hipGetLastError());
@@ -1220,7 +1248,7 @@ public:
         const int nbatch = dft_pending;
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
-            hipLaunchKernelGGL((k_dft<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, ports_all,
+            CFD_LAUNCH((k_dft<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, ports_all,
                                nports, nbatch, nullptr, dft_first, a.m.dft_stride, a.twE, a.twH, (int)a.m.freqs.size(), a.m.nt);
         });
         HIP_OK(hipGetLastError());
@@ -1395,31 +1423,31 @@ public:
         const int nm = (int)mem.size();
         const dim3 blk(256);
         if (nworkH) timer.run(stream, "pack_h", [&] {
-            if (nt_fields) hipLaunchKernelGGL((k_pack_h<R, true>), dim3(nworkH), blk, 0, stream, dmem, workH, nstep, q);
-            else hipLaunchKernelGGL((k_pack_h<R, false>), dim3(nworkH), blk, 0, stream, dmem, workH, nstep, q);
+            if (nt_fields) CFD_LAUNCH((k_pack_h<R, true>), dim3(nworkH), blk, 0, stream, dmem, workH, nstep, q);
+            else CFD_LAUNCH((k_pack_h<R, false>), dim3(nworkH), blk, 0, stream, dmem, workH, nstep, q);
         });
         if (nb_tfH) timer.run(stream, "pack_tfsf_h", [&] {
-            hipLaunchKernelGGL(k_pack_tfsf<R>, dim3(nb_tfH), blk, 0, stream, dmem, st_tfH, nm, nstep, q, 1);
+            CFD_LAUNCH(k_pack_tfsf<R>, dim3(nb_tfH), blk, 0, stream, dmem, st_tfH, nm, nstep, q, 1);
         });
         if (nworkE) timer.run(stream, "pack_e", [&] {
-            if (nt_fields) hipLaunchKernelGGL((k_pack_e<R, true>), dim3(nworkE), blk, 0, stream, dmem, workE, nstep, q);
-            else hipLaunchKernelGGL((k_pack_e<R, false>), dim3(nworkE), blk, 0, stream, dmem, workE, nstep, q);
+            if (nt_fields) CFD_LAUNCH((k_pack_e<R, true>), dim3(nworkE), blk, 0, stream, dmem, workE, nstep, q);
+            else CFD_LAUNCH((k_pack_e<R, false>), dim3(nworkE), blk, 0, stream, dmem, workE, nstep, q);
         });
         if (nb_sp) timer.run(stream, "pack_special_e", [&] {
-            hipLaunchKernelGGL(k_pack_special<R>, dim3(nb_sp), blk, 0, stream, dmem, st_sp, nm, nstep, q);
+            CFD_LAUNCH(k_pack_special<R>, dim3(nb_sp), blk, 0, stream, dmem, st_sp, nm, nstep, q);
         });
         if (nb_tfE) timer.run(stream, "pack_tfsf_e", [&] {
-            hipLaunchKernelGGL(k_pack_tfsf<R>, dim3(nb_tfE), blk, 0, stream, dmem, st_tfE, nm, nstep, q, 0);
+            CFD_LAUNCH(k_pack_tfsf<R>, dim3(nb_tfE), blk, 0, stream, dmem, st_tfE, nm, nstep, q, 0);
         });
         if (nb_src) timer.run(stream, "pack_sources", [&] {
-            hipLaunchKernelGGL(k_pack_sources<R>, dim3(nb_src), dim3(64), 0, stream, dmem, st_src, nm, nstep, q);
+            CFD_LAUNCH(k_pack_sources<R>, dim3(nb_src), dim3(64), 0, stream, dmem, st_src, nm, nstep, q);
         });
         if (nb_port) {
             timer.run(stream, "pack_ports", [&] {
-                hipLaunchKernelGGL(k_pack_port<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
+                CFD_LAUNCH(k_pack_port<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
             });
             if (any_dft) timer.run(stream, "pack_dft", [&] {
-                hipLaunchKernelGGL(k_pack_dft<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
+                CFD_LAUNCH(k_pack_dft<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
             });
         }
         HIP_OK(hipGetLastError());
@@ -1427,7 +1455,7 @@ public:
     void launch_sequence(int count) {
         for (int q = 0; q < count; q++) launch_step(q);
         const int nm = (int)mem.size();
-        hipLaunchKernelGGL(k_pack_advance, dim3((nm + 63) / 64), dim3(64), 0, stream, nstep, nm, count);
+        CFD_LAUNCH(k_pack_advance, dim3((nm + 63) / 64), dim3(64), 0, stream, nstep, nm, count);
         HIP_OK(hipGetLastError());
     }
     void capture() {
@@ -1452,8 +1480,8 @@ public:
         for (int q = 0; q < n; q++) {
             Sim<R> *s = dynamic_cast<Sim<R> *>(sims[q]);
             if (!s || std::find(mem.begin(), mem.end(), s) == mem.end()) throw Error("pack: not a member");
-            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, stream, s->count_dev, 0);
-            hipLaunchKernelGGL(k_nonfinite<R>, dim3(1024), dim3(256), 0, stream, s->d, s->d.slab * (s->m.Nz + 1),
+            CFD_LAUNCH(k_set_int, dim3(1), dim3(64), 0, stream, s->count_dev, 0);
+            CFD_LAUNCH(k_nonfinite<R>, dim3(1024), dim3(256), 0, stream, s->d, s->d.slab * (s->m.Nz + 1),
                                s->count_dev);
             ms.push_back(s);
         }
