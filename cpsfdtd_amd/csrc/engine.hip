@@ -385,18 +385,31 @@ public:
         d.cpmlH = (R)m.cpmlH;
         d.hsign_fixed = (m.flags & CFD_FIX_CPML_HSIGN) ? 1 : 0;
         const int N[3] = {m.Nx, m.Ny, m.Nz};
-        for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++) {
+            size_t box[2] = {0, 0};
             for (int e = 0; e < 2; e++) {
-                const int n = m.npml[a][e];
-                d.npml[a][e] = n;
-                if (!n) continue;
-                d.pb[0][a][e] = upload<R>(m.pbE[a][e]); d.pc[0][a][e] = upload<R>(m.pcE[a][e]);
-                d.pb[1][a][e] = upload<R>(m.pbH[a][e]); d.pc[1][a][e] = upload<R>(m.pcH[a][e]);
-                size_t box = a == 0 ? (size_t)n * d.py * (N[2] + 1) : a == 1 ? (size_t)d.px * n * (N[2] + 1)
-                                                                            : (size_t)d.slab * n;
-                for (int h = 0; h < 2; h++)
-                    for (int w = 0; w < 2; w++) d.q[h][a][e][w] = alloc<R>(box);
+                const size_t n = (size_t)m.npml[a][e];
+                d.npml[a][e] = (int)n;
+                box[e] = a == 0 ? n * d.py * (N[2] + 1) : a == 1 ? (size_t)d.px * n * (N[2] + 1) : (size_t)d.slab * n;
             }
+            if (!box[0] && !box[1]) continue;
+            // low side then high side in one array (coefficients) or box (psi)
+            auto cat = [](const std::vector<double> &lo, const std::vector<double> &hi) {
+                std::vector<double> v(lo);
+                v.insert(v.end(), hi.begin(), hi.end());
+                return v;
+            };
+            d.pb[0][a] = upload<R>(cat(m.pbE[a][0], m.pbE[a][1]));
+            d.pc[0][a] = upload<R>(cat(m.pcE[a][0], m.pcE[a][1]));
+            d.pb[1][a] = upload<R>(cat(m.pbH[a][0], m.pbH[a][1]));
+            d.pc[1][a] = upload<R>(cat(m.pcH[a][0], m.pcH[a][1]));
+            d.pofs[0][a] = (int)m.pbE[a][0].size();
+            d.pofs[1][a] = (int)m.pbH[a][0].size();
+            for (int h = 0; h < 2; h++) {
+                d.qofs[h][a] = (long long)box[0];
+                for (int w = 0; w < 2; w++) d.q[h][a][w] = alloc<R>(box[0] + box[1]);
+            }
+        }
         // Drude-Lorentz and gain cells: slots numbered in device order (x fastest),
         // so the lanes of a wavefront touch consecutive state words
         if (m.has_dl || m.has_gain) {

@@ -45,8 +45,14 @@ struct Dev {
     // CPML
     int has_pml;
     int npml[3][2];
-    const R *pb[2][3][2], *pc[2][3][2];  // [E/H][axis][side]
-    R *q[2][3][2][2];                    // [E/H][axis][side][which]
+    // CPML slabs per [E/H][axis]: the low side's grading coefficients then the
+    // high side's in one array (high side from index pofs), and each psi
+    // array holding the low slab's box then the high slab's (from qofs), so
+    // a cell's side selects an offset from a uniform base, not a pointer
+    const R *pb[2][3], *pc[2][3];
+    R *q[2][3][2];                       // [E/H][axis][which]
+    int pofs[2][3];
+    long long qofs[2][3];
     R cpmlE, cpmlH;
     int hsign_fixed;
     // Drude-Lorentz

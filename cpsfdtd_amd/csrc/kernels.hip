@@ -64,7 +64,7 @@ __device__ __forceinline__ bool special_id(const Dev<R> &d, int id)
 template <class R>
 struct SlabTerm {
     R b, c, q1, q2;  // grading coefficients and the two psi values
-    long long qi;    // psi index in the slab box
+    long long qi;    // psi index in the axis's arrays (the high slab's box follows the low one's)
     int side;        // -1: outside both slabs; 0: low side; 1: high side
 };
 
@@ -97,15 +97,17 @@ __device__ __forceinline__ void gather_slabs(const Dev<R> &d, int f, unsigned pm
         const long long qi = ax == 0 ? l + (long long)n * (j + (long long)d.py * k)
                            : ax == 1 ? i + (long long)d.px * (l + (long long)n * k)
                                      : i + (long long)d.px * (j + (long long)d.py * l);
-        const R *pb = side ? d.pb[f][ax][1] : d.pb[f][ax][0];
-        const R *pc = side ? d.pc[f][ax][1] : d.pc[f][ax][0];
-        const R *q1 = side ? d.q[f][ax][1][0] : d.q[f][ax][0][0];
-        const R *q2 = side ? d.q[f][ax][1][1] : d.q[f][ax][0][1];
+        const int cb = side ? d.pofs[f][ax] : 0;
+        const long long qb = side ? d.qofs[f][ax] : 0;
+        const R *pb = d.pb[f][ax] + cb;
+        const R *pc = d.pc[f][ax] + cb;
+        const R *q1 = d.q[f][ax][0] + qb;
+        const R *q2 = d.q[f][ax][1] + qb;
         T[ax].b = pb[ci];
         T[ax].c = pc[ci];
         T[ax].q1 = q1[qi];
         T[ax].q2 = q2[qi];
-        T[ax].qi = qi;
+        T[ax].qi = qb + qi;  // index in the axis's psi arrays
         T[ax].side = side;
     }
 }
@@ -113,10 +115,8 @@ __device__ __forceinline__ void gather_slabs(const Dev<R> &d, int f, unsigned pm
 template <class R>
 __device__ __forceinline__ void slab_store(const Dev<R> &d, int f, int ax, const SlabTerm<R> &t, R a1, R a2)
 {
-    R *q1 = t.side ? d.q[f][ax][1][0] : d.q[f][ax][0][0];
-    R *q2 = t.side ? d.q[f][ax][1][1] : d.q[f][ax][0][1];
-    q1[t.qi] = a1;
-    q2[t.qi] = a2;
+    d.q[f][ax][0][t.qi] = a1;
+    d.q[f][ax][1][t.qi] = a2;
 }
 
 // H at one cell: BasicFDTD.cpp:67-81, then CPML.cpp:609-689 (x, then y, then
