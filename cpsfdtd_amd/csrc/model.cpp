@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <unordered_map>
 #include <sstream>
 
 namespace cfd {
@@ -87,14 +88,20 @@ public:
                         m_.id[m_.flat(i3, j3, k3)] = (int16_t)s.material;
                 }
     }
-    // Structure.cpp:205-221
+    // Structure.cpp:205-221.  The three terms are non-negative (or NaN), and a
+    // rounded sum of non-negative terms is never below either term, so a row
+    // (i) or column (i, j) whose partial sum already exceeds 1 (or is NaN)
+    // holds no painted cell: skipping it leaves the painted set unchanged and
+    // scans a sphere's bounding columns instead of the whole grid.
     void ellipsoid(const ShapeRec &s) {
         const double dx = m_.dx, dy = m_.dy, dz = m_.dz;
         for (int i = 0; i < m_.Nx; i++) {
             double tx = ((double)i * dx - s.x0) * ((double)i * dx - s.x0) / (s.a * s.a);
+            if (!(tx <= 1.0)) continue;
             for (int j = 0; j < m_.Ny; j++) {
                 double ty = ((double)j * dy - s.y0) * ((double)j * dy - s.y0) / (s.b * s.b);
                 double txy = tx + ty;
+                if (!(txy <= 1.0)) continue;
                 for (int k = 0; k < m_.Nz; k++) {
                     double t = txy + ((double)k * dz - s.z0) * ((double)k * dz - s.z0) / (s.c * s.c);
                     if (t <= 1.0) m_.id[m_.flat(i, j, k)] = (int16_t)s.material;
@@ -102,16 +109,19 @@ public:
             }
         }
     }
-    // Structure.cpp:223-236
+    // Structure.cpp:223-236 (rows whose x term alone exceeds 1 skipped, as
+    // for the ellipsoid)
     void cylinder(const ShapeRec &s) {
         const double dx = m_.dx, dy = m_.dy;
-        for (int i = 0; i < m_.Nx; i++)
+        for (int i = 0; i < m_.Nx; i++) {
+            const double tx = ((double)i * dx - s.x0) * ((double)i * dx - s.x0) / (s.a * s.a);
+            if (!(tx <= 1.0)) continue;
             for (int j = 0; j < m_.Ny; j++) {
-                double t = ((double)i * dx - s.x0) * ((double)i * dx - s.x0) / (s.a * s.a) +
-                           ((double)j * dy - s.y0) * ((double)j * dy - s.y0) / (s.b * s.b);
+                double t = tx + ((double)j * dy - s.y0) * ((double)j * dy - s.y0) / (s.b * s.b);
                 if (t <= 1.0)
                     for (int k = 0; k < s.kc; k++) set(i, j, k + s.kz0, s.material);
             }
+        }
     }
     // Structure.cpp:238-306
     void multi(const ShapeRec &multi, const ShapeRec &single) {
@@ -418,24 +428,27 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
     const double sgn[3][2] = {{+1, -1}, {-1, +1}, {+1, -1}};
     const int far = (m.flags & CFD_FIX_TFSF_FARFACE) ? 0 : 1;
 
-    std::map<std::pair<int, size_t>, int> eidx, hidx;
-    std::vector<std::vector<TfsfTerm>> eterms, hterms;
+    // targets in order of first appearance; terms kept flat with their
+    // target's number, grouped per target (stable) once every face is done
+    std::unordered_map<unsigned long long, int> eidx, hidx;  // key: cell * 8 + component
+    std::vector<int> eown, hown;
+    std::vector<TfsfTerm> eterms, hterms;
+    {
+        const size_t nrec = 2 * ((size_t)(i2 - i1 + 1) * (k2 - k1 + 1) + (size_t)(j2 - j1 + 1) * (k2 - k1 + 1) +
+                                 (size_t)(i2 - i1 + 1) * (j2 - j1 + 1));
+        eidx.reserve(2 * nrec); hidx.reserve(2 * nrec);
+        eown.reserve(2 * nrec); hown.reserve(2 * nrec);
+        eterms.reserve(2 * nrec); hterms.reserve(2 * nrec);
+        m.tf_face_terms.reserve(4 * nrec);
+    }
     auto add = [&](bool h, int comp, int i, int j, int k, double coef, double delay, double prof) {
         auto &idx = h ? hidx : eidx;
         auto &tg = h ? m.tfH : m.tfE;
-        auto &terms = h ? hterms : eterms;
-        const auto key = std::make_pair(comp, m.flat(i, j, k));
-        auto it = idx.find(key);
-        int t;
-        if (it == idx.end()) {
-            t = (int)tg.size();
-            idx.emplace(key, t);
-            tg.push_back({comp, i, j, k, 0, 0});
-            terms.emplace_back();
-        } else {
-            t = it->second;
-        }
-        terms[t].push_back({coef, delay, prof});
+        const unsigned long long key = (unsigned long long)m.flat(i, j, k) * 8u + (unsigned)comp;
+        const auto ins = idx.emplace(key, (int)tg.size());
+        if (ins.second) tg.push_back({comp, i, j, k, 0, 0});
+        (h ? hown : eown).push_back(ins.first->second);
+        (h ? hterms : eterms).push_back({coef, delay, prof});
     };
     // a face alone: its records' corrections, each target once; the z1
     // variants leave out the transverse profile (TF_SF.cpp:1126-1143 and
@@ -499,14 +512,24 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
             }
         }
     }
+    m.tf_terms.resize(eterms.size() + hterms.size());
+    size_t base = 0;
     for (int h = 0; h < 2; h++) {
         auto &tg = h ? m.tfH : m.tfE;
-        auto &terms = h ? hterms : eterms;
-        for (size_t t = 0; t < tg.size(); t++) {
-            tg[t].first = (int)m.tf_terms.size();
-            tg[t].count = (int)terms[t].size();
-            m.tf_terms.insert(m.tf_terms.end(), terms[t].begin(), terms[t].end());
+        const auto &terms = h ? hterms : eterms;
+        const auto &own = h ? hown : eown;
+        for (int t : own) tg[t].count++;
+        size_t o = base;
+        for (auto &g : tg) {
+            g.first = (int)o;
+            o += g.count;
+            g.count = 0;
         }
+        for (size_t q = 0; q < terms.size(); q++) {
+            TfsfTarget &g = tg[own[q]];
+            m.tf_terms[g.first + g.count++] = terms[q];
+        }
+        base = o;
     }
     if (nt_req <= 0) nt_req = Nt;
     tfsf_waveforms(m, lambda, dlambda, cw, std::max(Nt, nt_req));
