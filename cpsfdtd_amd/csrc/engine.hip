@@ -378,6 +378,12 @@ public:
             d.id = upload<uint8_t>(id);
         }
         d.aE = upload<R>(m.aE); d.bE = upload<R>(m.bE); d.aH = upload<R>(m.aH); d.bH = upload<R>(m.bH);
+        d.nsel = m.NoMedia <= kSelMat ? 1 : 0;
+        for (int q = 0; q < kSelMat; q++) {
+            const int s = q < m.NoMedia ? q : 0;  // IDs >= NoMedia never occur
+            d.selE[0][q] = (R)m.aE[s]; d.selE[1][q] = (R)m.bE[s];
+            d.selH[0][q] = (R)m.aH[s]; d.selH[1][q] = (R)m.bH[s];
+        }
         for (int t = 0; t < kNumStretch; t++) d.st[t] = upload<R>(m.stretch[t]);
         // CPML
         d.has_pml = m.has_pml;

@@ -19,6 +19,7 @@ namespace cfd {
 constexpr int kMaxPoles = 8;
 constexpr int TX = 64;  // tile width along x: one wavefront
 constexpr int TY = 4;   // tile height along y: wavefronts per workgroup
+constexpr int kSelMat = 8;  // media whose coefficients the kernels select without a table load
 
 // parts of the E half-step a kernel performs (fused step = all of them)
 enum : unsigned {
@@ -41,6 +42,11 @@ struct Dev {
     const uint8_t *id;
     const int *aux;
     const R *aE, *bE, *aH, *bH;
+    // with at most kSelMat media (nsel != 0) the same coefficients are also
+    // held here, in the kernel argument, and selected by the ID's bits: no
+    // table load that has to wait for the ID byte
+    int nsel;
+    R selE[2][kSelMat], selH[2][kSelMat];  // [a, b][ID]
     const R *st[12];
     // CPML
     int has_pml;

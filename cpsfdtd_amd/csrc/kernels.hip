@@ -59,6 +59,31 @@ __device__ __forceinline__ bool special_id(const Dev<R> &d, int id)
     return ((id == 2) & (d.ndl != 0)) | ((id == 5) & (d.ngain != 0));
 }
 
+// Coefficient of medium id.  FP32 with nsel: a three-level select on the ID's
+// bits over the values held in the kernel argument, so the update issues no
+// load that depends on the ID byte (a table load is a second memory round
+// trip after the field loads).  Otherwise the table: in FP64 the selects on
+// register pairs cost the shell and pack kernels a wave per SIMD.
+template <class R>
+__device__ __forceinline__ R mat_coef(const R (&v)[kSelMat], const R *tab, int nsel, int id)
+{
+    if (sizeof(R) == 4 && nsel) {
+        const bool b0 = id & 1, b1 = id & 2, b2 = id & 4;
+        const R v01 = b0 ? v[1] : v[0], v23 = b0 ? v[3] : v[2], v45 = b0 ? v[5] : v[4], v67 = b0 ? v[7] : v[6];
+        const R v03 = b1 ? v23 : v01, v47 = b1 ? v67 : v45;
+        return b2 ? v47 : v03;
+    }
+    return tab[id];
+}
+template <class R>
+__device__ __forceinline__ R coef_aE(const Dev<R> &d, int id) { return mat_coef(d.selE[0], d.aE, d.nsel, id); }
+template <class R>
+__device__ __forceinline__ R coef_bE(const Dev<R> &d, int id) { return mat_coef(d.selE[1], d.bE, d.nsel, id); }
+template <class R>
+__device__ __forceinline__ R coef_aH(const Dev<R> &d, int id) { return mat_coef(d.selH[0], d.aH, d.nsel, id); }
+template <class R>
+__device__ __forceinline__ R coef_bH(const Dev<R> &d, int id) { return mat_coef(d.selH[1], d.bH, d.nsel, id); }
+
 // CPML term of one axis at one cell.  The two slabs of an axis never overlap
 // (the model rejects n_low + n_high >= N), so at most one holds the cell.
 template <class R>
@@ -140,7 +165,7 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
     SlabTerm<R> T[3];
     gather_slabs(d, 1, pm, i, j, k, T);
     if (basic) {
-        const R aH = d.aH[id], bH = d.bH[id];
+        const R aH = coef_aH(d, id), bH = coef_bH(d, id);
         hx = aH * hx - bH * (ez_j1 - ez) * sxz + bH * (ey_k1 - ey) * sxy;
         hy = aH * hy - bH * (ex_k1 - ex) * syx + bH * (ez_i1 - ez) * syz;
         hz = aH * hz + bH * (ex_j1 - ex) * szx - bH * (ey_i1 - ey) * szy;
@@ -212,7 +237,7 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
         }
     }
     if (parts & E_BASIC) {  // branch-free: ID 4 zeroed, ID 2 left as is
-        const R aE = d.aE[id], bE = d.bE[id];
+        const R aE = coef_aE(d, id), bE = coef_bE(d, id);
         const R u[3] = {aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy,
                         aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz,
                         aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx};
@@ -248,7 +273,7 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
         }
     }
     if (gain && (gparts & (E_GAINE | E_GAINPOP))) {
-        const R coef = d.bE[id] / d.dt;
+        const R coef = coef_bE(d, id) / d.dt;
         double dot = 0.0;
         for (int c = 0; c < 3; c++) {
             const long long o = (long long)c * d.ngain + gs;
@@ -453,7 +478,7 @@ __global__ __launch_bounds__(TX *TY) void k_lean_h(Dev<R> d, Box B, LeanStretch<
     const int id = d.id[p];  // first: the material lookup then overlaps the field loads
     const R ex = d.F[0][p], ey = d.F[1][p], ez = d.F[2][p];
     R hx = ld_once<NT>(&d.F[3][p]), hy = ld_once<NT>(&d.F[4][p]), hz = ld_once<NT>(&d.F[5][p]);
-    const R aH = d.aH[id], bH = d.bH[id];
+    const R aH = coef_aH(d, id), bH = coef_bH(d, id);
     hx = aH * hx - bH * (d.F[2][pj1] - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (d.F[1][pk1] - ey) * lean_st<CS>(d, c, S_cHxy, k);
     hy = aH * hy - bH * (d.F[0][pk1] - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (d.F[2][pi1] - ez) * lean_st<CS>(d, c, S_cHyz, i);
     hz = aH * hz + bH * (d.F[0][pj1] - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (d.F[1][pi1] - ey) * lean_st<CS>(d, c, S_cHzy, i);
@@ -484,7 +509,7 @@ __global__ __launch_bounds__(TX *TY) void k_lean_e(Dev<R> d, Box B, LeanStretch<
     const R ex = ld_once<NT>(&d.F[0][p]), ey = ld_once<NT>(&d.F[1][p]), ez = ld_once<NT>(&d.F[2][p]);
     const R hz_jm = d.F[5][pjm], hy_km = d.F[4][pkm], hx_km = d.F[3][pkm];
     const R hz_im = d.F[5][pim], hy_im = d.F[4][pim], hx_jm = d.F[3][pjm];
-    const R aE = d.aE[id], bE = d.bE[id];
+    const R aE = coef_aE(d, id), bE = coef_bE(d, id);
     R e0 = aE * ex + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
     R e1 = aE * ey + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
     R e2 = aE * ez + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
@@ -596,7 +621,7 @@ __device__ __forceinline__ void e_special(const Dev<R> &d, int i, int j, int k, 
         }
     }
     {  // basic update: ID 2 left to the ADE below
-        const R aE = d.aE[id], bE = d.bE[id];
+        const R aE = coef_aE(d, id), bE = coef_bE(d, id);
         const R u[3] = {aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy,
                         aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz,
                         aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx};
@@ -627,7 +652,7 @@ __device__ __forceinline__ void e_special(const Dev<R> &d, int i, int j, int k, 
         }
     }
     if (gain) {  // E -= (bE/dt) dP, then the populations (FP64)
-        const R coef = d.bE[id] / d.dt;
+        const R coef = coef_bE(d, id) / d.dt;
         double dot = 0.0;
         for (int c = 0; c < 3; c++) {
             const R dP = Pnew[c] - P[c];
@@ -735,6 +760,13 @@ __device__ __forceinline__ const Dev<R> &instance(const Dev<R> &d0, const Dev<R>
 // the tiles so that each XCD takes a contiguous z-slab (which helps a
 // uniform grid) made both 12-22 us slower here.
 // ---------------------------------------------------------------------------
+// Waves per SIMD the merged kernels are built for: 8 in FP32, where the
+// register allocator keeps them without memory spills (scalar values that do
+// not fit the 8-wave budget go to vector-register lanes); FP64 left to the
+// allocator (its merged form runs 5 waves per SIMD).
+template <class R>
+constexpr int kFullWaves = sizeof(R) == 4 ? 8 : 1;
+
 struct MGrid {
     const unsigned *tiles;  // per plane: x0 | y0 << 15 | (32-wide ? 1 << 31 : 0)
     int per_plane;          // tiles per plane
@@ -801,7 +833,7 @@ __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Bo
     const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
     R hx = ld_once<NT>(&d.F[3][p]), hy = ld_once<NT>(&d.F[4][p]), hz = ld_once<NT>(&d.F[5][p]);
     if (tile_inside(L, t)) {
-        const R aH = d.aH[id], bH = d.bH[id];
+        const R aH = coef_aH(d, id), bH = coef_bH(d, id);
         hx = aH * hx - bH * (ez_j1 - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (ey_k1 - ey) * lean_st<CS>(d, c, S_cHxy, k);
         hy = aH * hy - bH * (ex_k1 - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (ez_i1 - ez) * lean_st<CS>(d, c, S_cHyz, i);
         hz = aH * hz + bH * (ex_j1 - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (ey_i1 - ey) * lean_st<CS>(d, c, S_cHzy, i);
@@ -815,7 +847,7 @@ __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Bo
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(256) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R>))) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance_z<R, B>(d0, devs);
     MTile t;
@@ -843,7 +875,7 @@ __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Bo
     // CPML tiles wait for the ID byte before issuing their psi loads)
     const R E0[3] = {E[0], E[1], E[2]};
     if (tile_inside(L, t)) {
-        const R aE = d.aE[id], bE = d.bE[id];
+        const R aE = coef_aE(d, id), bE = coef_bE(d, id);
         E[0] = aE * E[0] + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
         E[1] = aE * E[1] + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
         E[2] = aE * E[2] + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
@@ -861,7 +893,7 @@ __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Bo
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(256) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R>))) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance_z<R, B>(d0, devs);
     MTile t;
