@@ -399,21 +399,22 @@ public:
                 box[e] = a == 0 ? n * d.py * (N[2] + 1) : a == 1 ? (size_t)d.px * n * (N[2] + 1) : (size_t)d.slab * n;
             }
             if (!box[0] && !box[1]) continue;
-            // low side then high side in one array (coefficients) or box (psi)
-            auto cat = [](const std::vector<double> &lo, const std::vector<double> &hi) {
-                std::vector<double> v(lo);
-                v.insert(v.end(), hi.begin(), hi.end());
+            // low side then high side in one array of (b, c) pairs
+            // (coefficients) or of psi pairs (boxes)
+            auto pairs = [](const std::vector<double> &b0, const std::vector<double> &c0, const std::vector<double> &b1,
+                            const std::vector<double> &c1) {
+                std::vector<double> v;
+                for (size_t q = 0; q < b0.size(); q++) v.insert(v.end(), {b0[q], c0[q]});
+                for (size_t q = 0; q < b1.size(); q++) v.insert(v.end(), {b1[q], c1[q]});
                 return v;
             };
-            d.pb[0][a] = upload<R>(cat(m.pbE[a][0], m.pbE[a][1]));
-            d.pc[0][a] = upload<R>(cat(m.pcE[a][0], m.pcE[a][1]));
-            d.pb[1][a] = upload<R>(cat(m.pbH[a][0], m.pbH[a][1]));
-            d.pc[1][a] = upload<R>(cat(m.pcH[a][0], m.pcH[a][1]));
+            d.pbc[0][a] = reinterpret_cast<const Pair<R> *>(upload<R>(pairs(m.pbE[a][0], m.pcE[a][0], m.pbE[a][1], m.pcE[a][1])));
+            d.pbc[1][a] = reinterpret_cast<const Pair<R> *>(upload<R>(pairs(m.pbH[a][0], m.pcH[a][0], m.pbH[a][1], m.pcH[a][1])));
             d.pofs[0][a] = (int)m.pbE[a][0].size();
             d.pofs[1][a] = (int)m.pbH[a][0].size();
             for (int h = 0; h < 2; h++) {
                 d.qofs[h][a] = (long long)box[0];
-                for (int w = 0; w < 2; w++) d.q[h][a][w] = alloc<R>(box[0] + box[1]);
+                d.q[h][a] = alloc<Pair<R>>(box[0] + box[1]);
             }
         }
         // Drude-Lorentz and gain cells: slots numbered in device order (x fastest),

@@ -33,6 +33,12 @@ enum : unsigned {
 };
 enum : unsigned { H_BASIC = 1u, H_PML = 2u, H_ALL = 3u };
 
+// two values of one cell loaded or stored as one aligned 2-word access
+template <class R>
+struct alignas(2 * sizeof(R)) Pair {
+    R a, b;
+};
+
 template <class R>
 struct Dev {
     int Nx, Ny, Nz, px, py;
@@ -51,12 +57,14 @@ struct Dev {
     // CPML
     int has_pml;
     int npml[3][2];
-    // CPML slabs per [E/H][axis]: the low side's grading coefficients then the
-    // high side's in one array (high side from index pofs), and each psi
-    // array holding the low slab's box then the high slab's (from qofs), so
-    // a cell's side selects an offset from a uniform base, not a pointer
-    const R *pb[2][3], *pc[2][3];
-    R *q[2][3][2];                       // [E/H][axis][which]
+    // CPML slabs per [E/H][axis]: the low side's grading coefficient pairs
+    // (b, c) then the high side's in one array (high side from index pofs),
+    // and one psi array of pairs (the two psi of the axis at a cell, loaded
+    // and stored as one 2-word access) holding the low slab's box then the
+    // high slab's (from qofs), so a cell's side selects an offset from a
+    // uniform base, not a pointer
+    const Pair<R> *pbc[2][3];
+    Pair<R> *q[2][3];                    // [E/H][axis]
     int pofs[2][3];
     long long qofs[2][3];
     R cpmlE, cpmlH;

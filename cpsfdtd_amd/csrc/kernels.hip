@@ -124,15 +124,13 @@ __device__ __forceinline__ void gather_slabs(const Dev<R> &d, int f, unsigned pm
                                      : i + (long long)d.px * (j + (long long)d.py * l);
         const int cb = side ? d.pofs[f][ax] : 0;
         const long long qb = side ? d.qofs[f][ax] : 0;
-        const R *pb = d.pb[f][ax] + cb;
-        const R *pc = d.pc[f][ax] + cb;
-        const R *q1 = d.q[f][ax][0] + qb;
-        const R *q2 = d.q[f][ax][1] + qb;
-        T[ax].b = pb[ci];
-        T[ax].c = pc[ci];
-        T[ax].q1 = q1[qi];
-        T[ax].q2 = q2[qi];
-        T[ax].qi = qb + qi;  // index in the axis's psi arrays
+        const Pair<R> bc = d.pbc[f][ax][cb + ci];
+        const Pair<R> qq = d.q[f][ax][qb + qi];
+        T[ax].b = bc.a;
+        T[ax].c = bc.b;
+        T[ax].q1 = qq.a;
+        T[ax].q2 = qq.b;
+        T[ax].qi = qb + qi;  // index in the axis's psi array
         T[ax].side = side;
     }
 }
@@ -140,8 +138,7 @@ __device__ __forceinline__ void gather_slabs(const Dev<R> &d, int f, unsigned pm
 template <class R>
 __device__ __forceinline__ void slab_store(const Dev<R> &d, int f, int ax, const SlabTerm<R> &t, R a1, R a2)
 {
-    d.q[f][ax][0][t.qi] = a1;
-    d.q[f][ax][1][t.qi] = a2;
+    d.q[f][ax][t.qi] = Pair<R>{a1, a2};
 }
 
 // H at one cell: BasicFDTD.cpp:67-81, then CPML.cpp:609-689 (x, then y, then
@@ -870,23 +867,27 @@ __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Bo
     const R hx = d.F[3][p], hy = d.F[4][p], hz = d.F[5][p];
     const R hx_jm = d.F[3][pjm], hz_jm = d.F[5][pjm], hx_km = d.F[3][pkm];
     const R hy_km = d.F[4][pkm], hy_im = d.F[4][pim], hz_im = d.F[5][pim];
-    // the special pass's cells keep their values here; the ID test sits in
-    // each branch after its loads (hoisted ahead of the branch, it made the
-    // CPML tiles wait for the ID byte before issuing their psi loads)
-    const R E0[3] = {E[0], E[1], E[2]};
+    // The special pass's cells keep their values: every cell is computed and
+    // theirs are not stored.  (Restoring their old values with a select made
+    // the compiler branch around the arithmetic, and the branch's default
+    // values waited for the E loads before the stretch constants were
+    // fetched.)  The ID test sits in each branch after its loads (hoisted
+    // ahead of the branch, it made the CPML tiles wait for the ID byte before
+    // issuing their psi loads).
+    bool keep;
     if (tile_inside(L, t)) {
         const R aE = coef_aE(d, id), bE = coef_bE(d, id);
         E[0] = aE * E[0] + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
         E[1] = aE * E[1] + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
         E[2] = aE * E[2] + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
-        const bool keep = special_id(d, id) | (id == 2);  // ID 2 without the ADE module is left as is
-        for (int q = 0; q < 3; q++) E[q] = keep ? E0[q] : id == 4 ? (R)0 : E[q];
+        for (int q = 0; q < 3; q++) E[q] = id == 4 ? (R)0 : E[q];
+        keep = special_id(d, id) | (id == 2);  // ID 2 without the ADE module is left as is
     } else {
         e_cell(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, t, true) : 0u, i, j, k, p, id, hx, hy, hz, hx_jm,
                hz_jm, hx_km, hy_km, hy_im, hz_im, E, true);
-        const bool sp = special_id(d, id);
-        for (int q = 0; q < 3; q++) E[q] = sp ? E0[q] : E[q];
+        keep = special_id(d, id);
     }
+    if (keep) return;
     st_once<NT>(&d.F[0][p], E[0]);
     st_once<NT>(&d.F[1][p], E[1]);
     st_once<NT>(&d.F[2][p], E[2]);
