@@ -574,7 +574,9 @@ public:
     // y_org: the first tile row (the E pass starts at its first updated row,
     // j = 1 without y periodicity: row 0 would idle a quarter of the first
     // row of tiles; measured 142.5 -> 139.6 us for k_full_e on 256^3)
-    std::vector<unsigned> mgrid_tiles(int y_org = 0) const {
+    // efield, L: the half-step the table is for and its interior box (the
+    // flags of each tile, kTile*)
+    std::vector<MTileEntry> mgrid_tiles(int y_org, bool efield, const Box &L) const {
         const char *v = std::getenv("CFD_TILE_EDGE32");
         const bool edge32 = !v || std::string(v) != "0";
         const int n0 = edge32 ? m.npml[0][0] : 0, n1 = edge32 ? m.npml[0][1] : 0;
@@ -583,12 +585,21 @@ public:
         const int nw = n1 ? wide / 64 : (wide + 63) / 64;
         const int xb = xa + 64 * nw;
         if (m.Nx > 0x7fff || m.Ny > 0xffff) throw Error("grid too large for the merged tile table");
-        std::vector<unsigned> t;
+        const int o = efield ? 1 : 0;  // E slabs start at index 1 on the low side
+        auto entry = [&](int x0, int y0, bool narrow) {
+            const int tw = narrow ? 32 : 64, th = narrow ? 8 : 4;
+            unsigned f = 0;
+            if ((m.npml[0][0] && x0 < m.npml[0][0] + o) || (m.npml[0][1] && x0 + tw > m.Nx - m.npml[0][1])) f |= kTileSlabX;
+            if ((m.npml[1][0] && y0 < m.npml[1][0] + o) || (m.npml[1][1] && y0 + th > m.Ny - m.npml[1][1])) f |= kTileSlabY;
+            if (x0 >= L.i0 && x0 + tw <= L.i1 && y0 >= L.j0 && y0 + th <= L.j1) f |= kTileInnerXY;
+            return MTileEntry{(unsigned)x0 | (unsigned)y0 << 15 | (narrow ? 1u << 31 : 0u), f};
+        };
+        std::vector<MTileEntry> t;
         for (int y0 = y_org; y0 < m.Ny; y0 += 4)
-            for (int b = 0; b < nw; b++) t.push_back((unsigned)(xa + 64 * b) | (unsigned)y0 << 15);
+            for (int b = 0; b < nw; b++) t.push_back(entry(xa + 64 * b, y0, false));
         for (int y0 = y_org; y0 < m.Ny; y0 += 8) {
-            for (int x0 = 0; x0 < xa; x0 += 32) t.push_back((unsigned)x0 | (unsigned)y0 << 15 | 1u << 31);
-            for (int x0 = xb; x0 < m.Nx; x0 += 32) t.push_back((unsigned)x0 | (unsigned)y0 << 15 | 1u << 31);
+            for (int x0 = 0; x0 < xa; x0 += 32) t.push_back(entry(x0, y0, true));
+            for (int x0 = xb; x0 < m.Nx; x0 += 32) t.push_back(entry(x0, y0, true));
         }
         return t;
     }
@@ -691,9 +702,9 @@ public:
         leanH_cs = lean_constant(leanH, false, leanH_st);
         leanE_cs = lean_constant(leanE, true, leanE_st);
         {
-            const std::vector<unsigned> t = mgrid_tiles(0), te = mgrid_tiles(m.pby ? 0 : 1);
-            mgH = MGrid{upload<unsigned>(t), (int)t.size(), 0, m.Nz};
-            mgE = MGrid{upload<unsigned>(te), (int)te.size(), 1, std::max(0, m.Nz - 1)};
+            const std::vector<MTileEntry> t = mgrid_tiles(0, false, leanH), te = mgrid_tiles(m.pby ? 0 : 1, true, leanE);
+            mgH = MGrid{upload<MTileEntry>(t), (int)t.size(), 0, m.Nz};
+            mgE = MGrid{upload<MTileEntry>(te), (int)te.size(), 1, std::max(0, m.Nz - 1)};
         }
         // every Drude-Lorentz / gain cell, in device order; cells outside the
         // interior box carry the CPML flag

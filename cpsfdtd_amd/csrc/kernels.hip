@@ -764,21 +764,36 @@ __device__ __forceinline__ const Dev<R> &instance(const Dev<R> &d0, const Dev<R>
 template <class R>
 constexpr int kFullWaves = sizeof(R) == 4 ? 8 : 1;
 
+// One tile of a plane: its origin, and what the host found about it for the
+// half-step the table belongs to (so the kernel branches on one loaded word
+// instead of comparing the tile against the slabs and the interior box).
+enum : unsigned {
+    kTileSlabX = 1u,   // reaches an x CPML slab
+    kTileSlabY = 2u,   // reaches a y CPML slab
+    kTileInnerXY = 4u  // lies in the interior box's x and y range
+};
+struct alignas(8) MTileEntry {
+    unsigned org;    // x0 | y0 << 15 | (32-wide ? 1 << 31 : 0)
+    unsigned flags;  // kTile*
+};
 struct MGrid {
-    const unsigned *tiles;  // per plane: x0 | y0 << 15 | (32-wide ? 1 << 31 : 0)
-    int per_plane;          // tiles per plane
-    int k_lo, planes;       // planes [k_lo, k_lo + planes): H from 0, E from 1
+    const MTileEntry *tiles;  // one plane's tiles
+    int per_plane;            // tiles per plane
+    int k_lo, planes;         // planes [k_lo, k_lo + planes): H from 0, E from 1
 };
 
 struct MTile {
     int x0, y0, k;   // tile origin and plane
     int tw, th;      // 64 x 4 or 32 x 8
     int i, j;        // this thread's cell
+    unsigned flags;  // kTile*
 };
 // tile `tile` of the plane table at plane k (absolute)
 __device__ __forceinline__ void mtile_at(const MGrid &g, unsigned tile, int k, MTile &t)
 {
-    const unsigned e = g.tiles[tile];
+    const MTileEntry te = g.tiles[tile];
+    const unsigned e = te.org;
+    t.flags = te.flags;
     const bool narrow = e >> 31;
     t.x0 = (int)(e & 0x7fffu);
     t.y0 = (int)((e >> 15) & 0xffffu);
@@ -799,15 +814,13 @@ __device__ __forceinline__ const Dev<R> &instance_z(const Dev<R> &d0, const Dev<
 
 __device__ __forceinline__ bool tile_inside(const Box &L, const MTile &t)
 {
-    return t.x0 >= L.i0 && t.x0 + t.tw <= L.i1 && t.y0 >= L.j0 && t.y0 + t.th <= L.j1 && t.k >= L.k0 && t.k < L.k1;
+    return (t.flags & kTileInnerXY) && t.k >= L.k0 && t.k < L.k1;
 }
 template <class R>
 __device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, const MTile &t, bool efield)
 {
     const int o = efield ? 1 : 0;  // E slabs start at index 1 on the low side
-    unsigned m = 0;
-    if ((d.npml[0][0] && t.x0 < d.npml[0][0] + o) || (d.npml[0][1] && t.x0 + t.tw > d.Nx - d.npml[0][1])) m |= 1u;
-    if ((d.npml[1][0] && t.y0 < d.npml[1][0] + o) || (d.npml[1][1] && t.y0 + t.th > d.Ny - d.npml[1][1])) m |= 2u;
+    unsigned m = t.flags & (kTileSlabX | kTileSlabY);
     if ((d.npml[2][0] && t.k < d.npml[2][0] + o) || (d.npml[2][1] && t.k >= d.Nz - d.npml[2][1])) m |= 4u;
     return m;
 }
