@@ -141,14 +141,41 @@ __device__ __forceinline__ void slab_store(const Dev<R> &d, int f, int ax, const
     d.q[f][ax][t.qi] = Pair<R>{a1, a2};
 }
 
+// Stretch factors of the interior box.  Outside the CPML slabs every entry of
+// each stretch table the lean kernels read is the same value (the host checks
+// this bit for bit over the box's index ranges); those kernels then take the
+// twelve values as scalars and skip six table loads per cell.
+template <class R>
+struct LeanStretch {
+    R s[12];
+};
+template <bool CS, class R>
+__device__ __forceinline__ R lean_st(const Dev<R> &d, const LeanStretch<R> &c, int t, int idx)
+{
+    return CS ? c.s[t] : d.st[t][idx];
+}
+
+// Stretch value of table t at index idx along axis ax in a CPML-path cell.
+// With CS (the tables are constant over the interior box), a tile whose slab
+// mask pm leaves ax out lies in the interior box's range along ax, where the
+// table holds that constant: no table load.
+template <bool CS, class R>
+__device__ __forceinline__ R st_at(const Dev<R> &d, const LeanStretch<R> *c, unsigned pm, int t, int ax, int idx)
+{
+    if constexpr (CS)
+        if (!(pm & (1u << ax))) return c->s[t];
+    return d.st[t][idx];
+}
+
 // H at one cell: BasicFDTD.cpp:67-81, then CPML.cpp:609-689 (x, then y, then
 // z slab contributions, in the reference's order).  *_i1/_j1/_k1 are E at the
 // +x/+y/+z neighbour.  Bit a of pm enables the slab test along axis a (a
 // caller passes 0 for an axis none of its cells can be in).
-template <class R>
+// With CS, cs holds the interior stretch constants (st_at).
+template <class R, bool CS = false>
 __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm, int i, int j, int k, long long p,
                                        R ex, R ey, R ez, R ex_j1, R ez_j1, R ex_k1, R ey_k1, R ey_i1, R ez_i1,
-                                       R &hx, R &hy, R &hz, int id_loaded = -1)
+                                       R &hx, R &hy, R &hz, int id_loaded = -1, const LeanStretch<R> *cs = nullptr)
 {
     // ID (unless the caller loaded it ahead of its field loads) and stretch
     // loads first, then the slab terms: every load of the cell is in flight
@@ -156,8 +183,9 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
     const int id = basic ? (id_loaded >= 0 ? id_loaded : (int)d.id[p]) : 0;
     R sxz = 0, sxy = 0, syx = 0, syz = 0, szx = 0, szy = 0;
     if (basic) {
-        sxz = d.st[S_cHxz][j]; sxy = d.st[S_cHxy][k]; syx = d.st[S_cHyx][k];
-        syz = d.st[S_cHyz][i]; szx = d.st[S_cHzx][j]; szy = d.st[S_cHzy][i];
+        sxz = st_at<CS>(d, cs, pm, S_cHxz, 1, j); sxy = st_at<CS>(d, cs, pm, S_cHxy, 2, k);
+        syx = st_at<CS>(d, cs, pm, S_cHyx, 2, k); syz = st_at<CS>(d, cs, pm, S_cHyz, 0, i);
+        szx = st_at<CS>(d, cs, pm, S_cHzx, 1, j); szy = st_at<CS>(d, cs, pm, S_cHzy, 0, i);
     }
     SlabTerm<R> T[3];
     gather_slabs(d, 1, pm, i, j, k, T);
@@ -199,16 +227,17 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
 // gain current and FP64 populations for ID 5, then CPML.cpp:529-607.
 // *_im/_jm/_km are H (already updated this step) at the -x/-y/-z neighbour;
 // pm as for h_cell.
-template <class R>
+template <class R, bool CS = false>
 __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned pm, int i, int j, int k, long long p,
                                        int id, R hx, R hy, R hz, R hx_jm, R hz_jm, R hx_km, R hy_km, R hy_im,
-                                       R hz_im, R (&E)[3], bool skip_special = false)
+                                       R hz_im, R (&E)[3], bool skip_special = false, const LeanStretch<R> *cs = nullptr)
 {
     // stretch values and slab terms loaded ahead of every store below
     R sxz = 0, sxy = 0, syx = 0, syz = 0, szy = 0, szx = 0;
     if (parts & (E_BASIC | E_DL)) {
-        sxz = d.st[S_cExz][j]; sxy = d.st[S_cExy][k]; syx = d.st[S_cEyx][k];
-        syz = d.st[S_cEyz][i]; szy = d.st[S_cEzy][i]; szx = d.st[S_cEzx][j];
+        sxz = st_at<CS>(d, cs, pm, S_cExz, 1, j); sxy = st_at<CS>(d, cs, pm, S_cExy, 2, k);
+        syx = st_at<CS>(d, cs, pm, S_cEyx, 2, k); syz = st_at<CS>(d, cs, pm, S_cEyz, 0, i);
+        szy = st_at<CS>(d, cs, pm, S_cEzy, 0, i); szx = st_at<CS>(d, cs, pm, S_cEzx, 1, j);
     }
     SlabTerm<R> T[3];
     gather_slabs(d, 0, pm, i, j, k, T);
@@ -446,19 +475,6 @@ __device__ __forceinline__ bool box_cell(const BoxSet &S, int &i, int &j, int &k
     return i >= B.i0 && i < B.i1 && j < B.j1;
 }
 
-// Stretch factors of the interior box.  Outside the CPML slabs every entry of
-// each stretch table the lean kernels read is the same value (the host checks
-// this bit for bit over the box's index ranges); those kernels then take the
-// twelve values as scalars and skip six table loads per cell.
-template <class R>
-struct LeanStretch {
-    R s[12];
-};
-template <bool CS, class R>
-__device__ __forceinline__ R lean_st(const Dev<R> &d, const LeanStretch<R> &c, int t, int idx)
-{
-    return CS ? c.s[t] : d.st[t][idx];
-}
 
 // interior H: BasicFDTD.cpp:67-81 only
 template <class R, bool CS, bool NT>
@@ -848,8 +864,8 @@ __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Bo
         hy = aH * hy - bH * (ex_k1 - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (ez_i1 - ez) * lean_st<CS>(d, c, S_cHyz, i);
         hz = aH * hz + bH * (ex_j1 - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (ey_i1 - ey) * lean_st<CS>(d, c, S_cHzy, i);
     } else {
-        h_cell(d, true, d.has_pml ? tile_pml_mask(d, t, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1, ex_k1,
-               ey_k1, ey_i1, ez_i1, hx, hy, hz, id);
+        h_cell<R, CS>(d, true, d.has_pml ? tile_pml_mask(d, t, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
+                      ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz, id, &c);
     }
     st_once<NT>(&d.F[3][p], hx);
     st_once<NT>(&d.F[4][p], hy);
@@ -896,8 +912,8 @@ __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Bo
         for (int q = 0; q < 3; q++) E[q] = id == 4 ? (R)0 : E[q];
         keep = special_id(d, id) | (id == 2);  // ID 2 without the ADE module is left as is
     } else {
-        e_cell(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, t, true) : 0u, i, j, k, p, id, hx, hy, hz, hx_jm,
-               hz_jm, hx_km, hy_km, hy_im, hz_im, E, true);
+        e_cell<R, CS>(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, t, true) : 0u, i, j, k, p, id, hx, hy, hz,
+                      hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E, true, &c);
         keep = special_id(d, id);
     }
     if (keep) return;
