@@ -1036,13 +1036,30 @@ public:
         const PortDesc<R> &p = dports.at(q);
         if (!p.dft) throw Error("no DFT frequencies configured");
         dft_flush();
-        const size_t nf = m.freqs.size(), np_ = p.npts;
-        std::vector<double> tmp(np_ * nf * 12);
-        HIP_OK(hipMemcpyAsync(tmp.data(), p.dft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        // device [f][c][re,im][point] -> host [point][f][c][re,im], transposed
+        // on the device in chunks of points (at most 32 MiB each) and copied
+        // straight into host
+        const int rows = (int)m.freqs.size() * 12;
+        const long long np_ = p.npts;
+        if (!np_) return;
+        const long long chunk = std::min(np_, std::max(32LL, (4LL << 20) / rows / 32 * 32));
+        double *buf = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(capture_mutex());
+            keep_pool_memory();
+            HIP_OK(hipMallocAsync(&buf, (size_t)chunk * rows * sizeof(double), stream));
+        }
+        for (long long t0 = 0; t0 < np_; t0 += chunk) {
+            const int n = (int)std::min(chunk, np_ - t0);
+            CFD_LAUNCH(k_dft_transpose, dim3((unsigned)((n + 31) / 32), (unsigned)((rows + 31) / 32)), dim3(256), 0, stream,
+                       p.dft, np_, rows, t0, n, buf);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(host + t0 * rows, buf, (size_t)n * rows * sizeof(double), hipMemcpyDeviceToHost, stream));
+        }
         HIP_OK(hipStreamSynchronize(stream));
-        for (size_t f = 0; f < nf; f++)  // device [f][c][re,im][point] -> host [point][f][c][re,im]
-            for (size_t w = 0; w < 12; w++)
-                for (size_t t = 0; t < np_; t++) host[(t * nf + f) * 12 + w] = tmp[(f * 12 + w) * np_ + t];
+        std::lock_guard<std::mutex> lk(capture_mutex());
+        HIP_OK(hipFreeAsync(buf, stream));
+        HIP_OK(hipStreamSynchronize(stream));
     }
     void port_intensity(int q, double *host) override {
         const PortDesc<R> &p = dports.at(q);

@@ -1122,6 +1122,31 @@ struct PortDesc {
     int npts, nblk, bstart, pad;
 };
 
+// DFT accumulators of the points [t0, t0 + n) of a port, from the device
+// layout [row][point] (row = frequency * 12 + component * 2 + re/im, np
+// points per row) into the host layout [point][row]: 32 x 32 tiles through
+// LDS, so that reads and writes are both runs along their own rows.
+// Workgroup 32 x 8; grid (ceil(n / 32), ceil(rows / 32)).
+__global__ __launch_bounds__(256) void k_dft_transpose(const double *src, long long np, int rows, long long t0, int n,
+                                                       double *dst)
+{
+    __shared__ double tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int r0 = blockIdx.y * 32;
+    const long long c0 = (long long)blockIdx.x * 32;
+    for (int q = ty; q < 32; q += 8) {
+        const int r = r0 + q;
+        const long long c = c0 + tx;
+        if (r < rows && c < n) tile[q][tx] = src[(long long)r * np + t0 + c];
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const long long c = c0 + q;
+        const int r = r0 + tx;
+        if (r < rows && c < n) dst[c * rows + r] = tile[tx][q];
+    }
+}
+
 // Port gather (Ports.cpp:88-99, 342-353) into the time-series record and the
 // DFT batch, and the per-step sum of |E|^2 (Ports.cpp:512-522) reduced across
 // the workgroup with wavefront shuffles into one partial per workgroup.

@@ -73,6 +73,14 @@ struct ConfigCopy {
     }
 };
 
+// A result array filled by a device read-back (no zero fill first: a
+// plane port's per-point DFT runs to hundreds of MB).
+struct ResultBuf {
+    std::unique_ptr<double[]> v;
+    size_t n = 0;
+    explicit ResultBuf(size_t count = 0) : v(count ? new double[count] : nullptr), n(count) {}
+};
+
 struct Request {
     long long id = 0;
     int state = CFD_REQ_QUEUED;
@@ -89,7 +97,7 @@ struct Request {
     int pack = 0;               // which pack
     double t_submit = 0, t_start = 0, t_done = 0;
     std::string error;
-    std::vector<std::vector<double>> intensity, dft;  // per port
+    std::vector<ResultBuf> intensity, dft;  // per port
 };
 
 }  // namespace
@@ -127,12 +135,12 @@ struct cfd_server {
             cfd_info(r.sim, nullptr, nullptr, counts);
             for (int q = 0; q < counts[0] && ok; q++) {
                 const int npts = cfd_port_npts(r.sim, q);
-                std::vector<double> in((size_t)r.nt), df;
-                ok = cfd_port_intensity(r.sim, q, in.data()) >= 0;
+                ResultBuf in((size_t)r.nt), df;
+                ok = cfd_port_intensity(r.sim, q, in.v.get()) >= 0;
                 const int nf = r.cfg->cfg.n_freq;
                 if (ok && nf > 0 && npts > 0) {
-                    df.resize((size_t)npts * nf * 12);
-                    ok = cfd_port_dft(r.sim, q, df.data()) >= 0;
+                    df = ResultBuf((size_t)npts * nf * 12);
+                    ok = cfd_port_dft(r.sim, q, df.v.get()) >= 0;
                 }
                 r.intensity.push_back(std::move(in));
                 r.dft.push_back(std::move(df));
@@ -530,9 +538,9 @@ int cfd_server_result(cfd_server *s, long long id, int port, int what, double *h
     if (r.state == CFD_REQ_FAILED) { g_server_error = "request failed: " + r.error; return -1; }
     if (r.state != CFD_REQ_DONE) { g_server_error = "request not finished"; return -1; }
     if (port < 0 || port >= (int)r.intensity.size()) { g_server_error = "no such port"; return -1; }
-    const std::vector<double> &v = what == CFD_RESULT_DFT ? r.dft[port] : r.intensity[port];
-    if (count) *count = v.size();
-    if (host && !v.empty()) std::memcpy(host, v.data(), v.size() * sizeof(double));
+    const ResultBuf &v = what == CFD_RESULT_DFT ? r.dft[port] : r.intensity[port];
+    if (count) *count = v.n;
+    if (host && v.n) std::memcpy(host, v.v.get(), v.n * sizeof(double));
     return 0;
 }
 
