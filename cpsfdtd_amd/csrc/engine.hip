@@ -557,6 +557,7 @@ public:
             port_tab = alloc<PortDesc<R>>(dports.size());
             HIP_OK(hipMemcpyAsync(port_tab, dports.data(), dports.size() * sizeof(PortDesc<R>), hipMemcpyHostToDevice, stream));
         }
+        setup_post();
         HIP_OK(hipStreamSynchronize(stream));
     }
 
@@ -624,6 +625,11 @@ public:
     long long *special = nullptr;
     int nspecial = 0;
     int spoles[2] = {-1, -1};  // special pass instantiation (launch_special)
+    // the passes after the E update in one launch (k_post_e), when their
+    // cells allow it (post_fusable)
+    bool post_fused = false, post_ports = false;  // post_ports: the port sampling is in it
+    PostE<R> post{};
+    int post_blocks = 0;
     bool nt_fields = false;  // streamed own-cell field traffic (see stream_fields)
 
     static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1, int tw) {
@@ -731,6 +737,8 @@ public:
         HIP_OK(hipStreamSynchronize(stream));
     }
 
+    // h: the H half-step, else E (and, unless the post-E launch does them,
+    // the special cells)
     void launch_update(bool h) {
         if (form == kFormGrid) {
             if (h) launch_h(H_ALL, "update_h");
@@ -747,7 +755,7 @@ public:
                 });
                 HIP_OK(hipGetLastError());
             }
-            if (!h && nspecial) {
+            if (!h && nspecial && !post_fused) {
                 timer.run(stream, "special_e", [&] {
                     launch_special<R, false>(stream, dim3((nspecial + 255) / 256), d, nullptr, special, nspecial, nullptr,
                                              spoles);
@@ -785,7 +793,7 @@ public:
             });
             HIP_OK(hipGetLastError());
         }
-        if (!h && nspecial) {
+        if (!h && nspecial && !post_fused) {
             timer.run(stream, "special_e", [&] {
                 launch_special<R, false>(stream, dim3((nspecial + 255) / 256), d, nullptr, special, nspecial, nullptr,
                                          spoles);
@@ -898,9 +906,114 @@ public:
         launch_update(true);
         tfsf(true, n);
         launch_update(false);
+        if (post_fused) {
+            post_e(n);
+            return;
+        }
         tfsf(false, n);
         sources(n);
         if (!dports.empty()) ports(n);
+    }
+    // special cells, TF/SF-E, sources and (post_ports) the port sampling of
+    // step n in one launch (k_post_e), with the port bookkeeping of ports();
+    // else the port sampling follows as its own launch
+    void post_e(int n) {
+        if (post_ports && (n < 0 || n >= m.nt)) throw Error("port time index out of range");
+        if (nsgroup && !nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
+        PostE<R> P = post;
+        P.slot = -1;
+        if (post_ports && !m.freqs.empty() && n % m.dft_stride == 0) {
+            if (dft_pending == 0) dft_first = n;
+            P.slot = dft_pending++;
+        }
+        timer.run(stream, "post_e", [&] {
+#define CFD_POST(ND, NL) CFD_LAUNCH((k_post_e<R, ND, NL>), dim3(post_blocks), dim3(256), 0, stream, d, P, nbase_cur, n)
+            if (!P.nspecial) CFD_POST(-2, -2);
+            else if (spoles[0] == 1 && spoles[1] == 2) CFD_POST(1, 2);
+            else if (spoles[0] == 1 && spoles[1] == 0) CFD_POST(1, 0);
+            else if (spoles[0] == 0 && spoles[1] == 0) CFD_POST(0, 0);
+            else CFD_POST(-1, -1);
+#undef CFD_POST
+        });
+        HIP_OK(hipGetLastError());
+        if (!post_ports) {
+            if (!dports.empty()) ports(n);
+            return;
+        }
+        if (dft_pending == kDftBatch) dft_flush();
+    }
+    // the post-E launch's tables and workgroup ranges, when post_fusable
+    // allows it (CFD_FUSE_POST=0: separate launches, for A/B runs)
+    void setup_post() {
+        post_fused = post_ports = false;
+        if (const char *v = std::getenv("CFD_FUSE_POST"))
+            if (std::string(v) == "0") return;
+        std::vector<long long> te, sp;
+        if (m.has_tfsf)
+            for (const TfsfTarget &t : m.tfE) te.push_back(dev_index(t.i, t.j, t.k));
+        const long long plane = (long long)(m.Ny + 1) * (m.Nz + 1);
+        for (const std::vector<long long> *cells : {&m.dl_cells, &m.gain_cells})
+            for (const long long f : *cells)
+                sp.push_back(dev_index((int)(f / plane), (int)((f / (m.Nz + 1)) % (m.Ny + 1)), (int)(f % (m.Nz + 1))));
+        std::vector<std::vector<long long>> pcs;
+        for (const Port &P : m.ports) {
+            pcs.emplace_back();
+            for (size_t q = 0; q < P.ijk.size(); q += 3) pcs.back().push_back(dev_index(P.ijk[q], P.ijk[q + 1], P.ijk[q + 2]));
+        }
+        const int level = post_fusable(te, sp, pcs);
+        if (!level) return;
+        post_ports = level == 2 && !dports.empty();
+        PostE<R> P{};
+        P.special = special;
+        P.nspecial = form == kFormGrid ? 0 : nspecial;
+        P.b1 = (P.nspecial + 255) / 256;
+        P.tg = tfE;
+        if constexpr (std::is_same<R, float>::value) P.tg32 = tf32 ? tfE32 : nullptr;
+        P.ntg = m.has_tfsf ? ntfE : 0;
+        P.b2 = P.b1 + (P.ntg + 255) / 256;
+        P.coef = tcoef; P.delay = tdelay; P.prof = tprof; P.tab = hinc;
+        if constexpr (std::is_same<R, float>::value) { P.terms32 = tterm32; P.tab32 = hinc32; }
+        P.tablen = tablen;
+        P.grp = sgroup;
+        P.ngroups = nsgroup;
+        P.b3 = P.b2 + (nsgroup + 255) / 256;
+        P.sidx = sidx;
+        P.swave = swave;
+        P.sstride = m.nt + 1;
+        P.ports = port_tab;
+        P.nports = post_ports ? (int)dports.size() : 0;
+        post = P;
+        post_blocks = P.b3 + (post_ports ? port_blocks : 0);
+        post_fused = post_blocks > 0;
+        if (!post_fused) post_ports = false;
+    }
+    // May the post-E passes run side by side?  The E values they write must
+    // lie at cells that no other of them writes (else 0); the port sampling
+    // joins them (2) only if it samples none of those cells, else it stays a
+    // launch of its own after them (1).  Cell-level test: conservative.
+    // te: TF/SF-E target cells; sp: special cells (flag bits included)
+    int post_fusable(std::vector<long long> te, const std::vector<long long> &sp,
+                     const std::vector<std::vector<long long>> &port_cells) const {
+        const long long mask = (1LL << kSpecialPml) - 1;
+        std::vector<long long> a, c, pc;
+        if (form != kFormGrid)
+            for (long long v : sp) a.push_back(v & mask);
+        for (int q = 0; q < (int)(m.src.size() / 4); q++) c.push_back(dev_index(m.src[4 * q], m.src[4 * q + 1], m.src[4 * q + 2]));
+        for (const std::vector<long long> &v : port_cells) pc.insert(pc.end(), v.begin(), v.end());
+        for (std::vector<long long> *v : {&a, &te, &c, &pc}) {
+            std::sort(v->begin(), v->end());
+            v->erase(std::unique(v->begin(), v->end()), v->end());
+        }
+        auto meet = [](const std::vector<long long> &x, const std::vector<long long> &y) {
+            size_t i = 0, j = 0;
+            while (i < x.size() && j < y.size()) {
+                if (x[i] == y[j]) return true;
+                if (x[i] < y[j]) i++; else j++;
+            }
+            return false;
+        };
+        if (meet(a, te) || meet(a, c) || meet(te, c)) return 0;
+        return meet(pc, a) || meet(pc, te) || meet(pc, c) ? 1 : 2;
     }
 
     // Step blocks replayed from a HIP graph: a block of graph_block() steps

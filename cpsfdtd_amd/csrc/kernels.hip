@@ -12,6 +12,8 @@
 // update (BasicFDTD.cpp, CPML.cpp, TF_SF.cpp) operation for operation.
 #pragma once
 
+#include <type_traits>
+
 #include "engine.hpp"
 
 namespace cfd {
@@ -939,8 +941,21 @@ struct SpecialList {
     const long long *cells;
     int n, pad;
 };
-// ND, NL >= 0: every instance's Drude-Lorentz cells have exactly ND Drude and
-// NL Lorentz poles (e_special); -1: any counts (e_full)
+// one entry v of the list; ND, NL >= 0: every instance's Drude-Lorentz cells
+// have exactly ND Drude and NL Lorentz poles (e_special); -1: any counts
+// (e_full)
+template <class R, int ND, int NL>
+__device__ __forceinline__ void special_cell(const Dev<R> &d, long long v)
+{
+    const long long p = v & ((1LL << kSpecialPml) - 1);
+    const int k = (int)(p / d.slab);
+    const long long r = p - (long long)k * d.slab;
+    const int j = (int)(r / d.px), i = (int)(r - (long long)j * d.px);
+    const unsigned pm = (v >> kSpecialPml) & 1 ? 7u : 0u;
+    if constexpr (ND < 0) e_full(d, i, j, k, pm);
+    else e_special<R, ND, NL>(d, i, j, k, pm);
+}
+
 template <class R, bool B = false, int ND = -1, int NL = -1>
 __global__ __launch_bounds__(256) void k_special_e(Dev<R> d0, const Dev<R> *devs, const long long *cells0, int n0,
                                                    const SpecialList *lists)
@@ -950,14 +965,7 @@ __global__ __launch_bounds__(256) void k_special_e(Dev<R> d0, const Dev<R> *devs
     const int n = B ? lists[blockIdx.y].n : n0;
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
-    const long long v = cells[t];
-    const long long p = v & ((1LL << kSpecialPml) - 1);
-    const int k = (int)(p / d.slab);
-    const long long r = p - (long long)k * d.slab;
-    const int j = (int)(r / d.px), i = (int)(r - (long long)j * d.px);
-    const unsigned pm = (v >> kSpecialPml) & 1 ? 7u : 0u;
-    if constexpr (ND < 0) e_full(d, i, j, k, pm);
-    else e_special<R, ND, NL>(d, i, j, k, pm);
+    special_cell<R, ND, NL>(d, cells[t]);
 }
 
 // TF_SF.cpp:966-986
@@ -1187,6 +1195,62 @@ __global__ __launch_bounds__(256) void k_port(Dev<R> d0, const Dev<R> *devs, con
     const Dev<R> &d = instance<R, B>(d0, devs);
     if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
     port_block(d, ports, nports, (int)blockIdx.x, n, slot);
+}
+
+// The passes after a step's E update in one launch, each on its own range of
+// workgroups: Drude-Lorentz / gain cells [0, b1), TF/SF-E corrections
+// [b1, b2), point sources [b2, b3), port sampling [b3, end).  Used only when
+// the cells whose E values these passes write are distinct from pass to pass
+// and the ports sample none of them (the host checks), so that running them
+// side by side gives every value the reference's sequence gives.  One launch
+// instead of four saves three ramp-ups and tails per step.
+template <class R>
+struct PostE {
+    const long long *special;
+    int nspecial, b1;
+    const TfTarget *tg;       // FP64 lists, or
+    const TfTarget32 *tg32;   // the compact FP32 ones when set
+    int ntg, b2;
+    const double *coef, *delay, *prof, *tab;
+    const TfTerm32 *terms32;
+    const float *tab32;
+    int tablen;
+    const SourceGroup *grp;
+    int ngroups, b3;
+    const int *sidx;
+    const double *swave;
+    int sstride;
+    const PortDesc<R> *ports;
+    int nports, slot;
+};
+// ND, NL as for special_cell; ND = -2: no special cells (the role compiled
+// out, so the other roles keep their register budget)
+template <class R, int ND, int NL>
+__global__ __launch_bounds__(256) void k_post_e(Dev<R> d, PostE<R> P, const int *nbase, int n)
+{
+    if (nbase) n += *nbase;  // replayed step blocks: step = block start + offset
+    const int b = blockIdx.x;
+    if (b < P.b1) {
+        if constexpr (ND >= -1) {
+            const int t = b * 256 + threadIdx.x;
+            if (t < P.nspecial) special_cell<R, ND, NL>(d, P.special[t]);
+        }
+    } else if (b < P.b2) {
+        const int t = (b - P.b1) * 256 + threadIdx.x;
+        if (t >= P.ntg) return;
+        if constexpr (std::is_same<R, float>::value) {
+            if (P.tg32) {
+                tfsf_apply32(d, P.tg32[t], P.terms32, P.tab32, P.tablen, n);
+                return;
+            }
+        }
+        tfsf_apply(d, P.tg[t], P.coef, P.delay, P.prof, P.tab, P.tablen, n);
+    } else if (b < P.b3) {
+        const int g = (b - P.b2) * 256 + threadIdx.x;
+        if (g < P.ngroups) source_apply(d, P.grp[g], P.sidx, P.swave, P.sstride, n);
+    } else {
+        port_block(d, P.ports, P.nports, b - P.b3, n, P.slot);  // the whole workgroup
+    }
 }
 
 // Running DFT over a batch of nb sampled steps n0, n0+stride, ...: each FP64
