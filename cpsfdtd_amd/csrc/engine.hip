@@ -1275,6 +1275,9 @@ public:
             int pl[3] = {0, 0, 0};
             for (auto &sp : sims) {
                 dv.push_back(sp->d);
+                // the batched kernels read this view from device memory: the
+                // coefficient tables, not 32 more scalars per workgroup
+                dv.back().nsel = 0;
                 sl.push_back(SpecialList{sp->special, sp->nspecial, 0});
                 max_special = std::max(max_special, sp->nspecial);
                 pa.insert(pa.end(), sp->dports.begin(), sp->dports.end());

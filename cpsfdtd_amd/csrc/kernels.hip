@@ -66,10 +66,12 @@ __device__ __forceinline__ bool special_id(const Dev<R> &d, int id)
 // load that depends on the ID byte (a table load is a second memory round
 // trip after the field loads).  Otherwise the table: in FP64 the selects on
 // register pairs cost the shell and pack kernels a wave per SIMD.
-template <class R>
+// SEL = false: always the table (kernels whose simulation view is read from
+// device memory, where the selected values would be 32 more scalar loads).
+template <bool SEL, class R>
 __device__ __forceinline__ R mat_coef(const R (&v)[kSelMat], const R *tab, int nsel, int id)
 {
-    if (sizeof(R) == 4 && nsel) {
+    if (SEL && sizeof(R) == 4 && nsel) {
         const bool b0 = id & 1, b1 = id & 2, b2 = id & 4;
         const R v01 = b0 ? v[1] : v[0], v23 = b0 ? v[3] : v[2], v45 = b0 ? v[5] : v[4], v67 = b0 ? v[7] : v[6];
         const R v03 = b1 ? v23 : v01, v47 = b1 ? v67 : v45;
@@ -77,14 +79,14 @@ __device__ __forceinline__ R mat_coef(const R (&v)[kSelMat], const R *tab, int n
     }
     return tab[id];
 }
-template <class R>
-__device__ __forceinline__ R coef_aE(const Dev<R> &d, int id) { return mat_coef(d.selE[0], d.aE, d.nsel, id); }
-template <class R>
-__device__ __forceinline__ R coef_bE(const Dev<R> &d, int id) { return mat_coef(d.selE[1], d.bE, d.nsel, id); }
-template <class R>
-__device__ __forceinline__ R coef_aH(const Dev<R> &d, int id) { return mat_coef(d.selH[0], d.aH, d.nsel, id); }
-template <class R>
-__device__ __forceinline__ R coef_bH(const Dev<R> &d, int id) { return mat_coef(d.selH[1], d.bH, d.nsel, id); }
+template <bool SEL = true, class R>
+__device__ __forceinline__ R coef_aE(const Dev<R> &d, int id) { return mat_coef<SEL>(d.selE[0], d.aE, d.nsel, id); }
+template <bool SEL = true, class R>
+__device__ __forceinline__ R coef_bE(const Dev<R> &d, int id) { return mat_coef<SEL>(d.selE[1], d.bE, d.nsel, id); }
+template <bool SEL = true, class R>
+__device__ __forceinline__ R coef_aH(const Dev<R> &d, int id) { return mat_coef<SEL>(d.selH[0], d.aH, d.nsel, id); }
+template <bool SEL = true, class R>
+__device__ __forceinline__ R coef_bH(const Dev<R> &d, int id) { return mat_coef<SEL>(d.selH[1], d.bH, d.nsel, id); }
 
 // CPML term of one axis at one cell.  The two slabs of an axis never overlap
 // (the model rejects n_low + n_high >= N), so at most one holds the cell.
@@ -181,7 +183,7 @@ __device__ __forceinline__ R st_at(const Dev<R> &d, const LeanStretch<R> *c, uns
 // caller passes 0 for an axis none of its cells can be in).
 // With CS, cs holds the interior stretch constants (st_at); NT as for
 // gather_slabs.
-template <class R, bool CS = false, bool NT = false>
+template <class R, bool CS = false, bool NT = false, bool SEL = true>
 __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm, int i, int j, int k, long long p,
                                        R ex, R ey, R ez, R ex_j1, R ez_j1, R ex_k1, R ey_k1, R ey_i1, R ez_i1,
                                        R &hx, R &hy, R &hz, int id_loaded = -1, const LeanStretch<R> *cs = nullptr)
@@ -199,7 +201,7 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
     SlabTerm<R> T[3];
     gather_slabs<R, NT>(d, 1, pm, i, j, k, T);
     if (basic) {
-        const R aH = coef_aH(d, id), bH = coef_bH(d, id);
+        const R aH = coef_aH<SEL>(d, id), bH = coef_bH<SEL>(d, id);
         hx = aH * hx - bH * (ez_j1 - ez) * sxz + bH * (ey_k1 - ey) * sxy;
         hy = aH * hy - bH * (ex_k1 - ex) * syx + bH * (ez_i1 - ez) * syz;
         hz = aH * hz + bH * (ex_j1 - ex) * szx - bH * (ey_i1 - ey) * szy;
@@ -236,7 +238,7 @@ __device__ __forceinline__ void h_cell(const Dev<R> &d, bool basic, unsigned pm,
 // gain current and FP64 populations for ID 5, then CPML.cpp:529-607.
 // *_im/_jm/_km are H (already updated this step) at the -x/-y/-z neighbour;
 // pm as for h_cell.
-template <class R, bool CS = false, bool NT = false>
+template <class R, bool CS = false, bool NT = false, bool SEL = true>
 __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned pm, int i, int j, int k, long long p,
                                        int id, R hx, R hy, R hz, R hx_jm, R hz_jm, R hx_km, R hy_km, R hy_im,
                                        R hz_im, R (&E)[3], bool skip_special = false, const LeanStretch<R> *cs = nullptr)
@@ -272,7 +274,7 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
         }
     }
     if (parts & E_BASIC) {  // branch-free: ID 4 zeroed, ID 2 left as is
-        const R aE = coef_aE(d, id), bE = coef_bE(d, id);
+        const R aE = coef_aE<SEL>(d, id), bE = coef_bE<SEL>(d, id);
         const R u[3] = {aE * E[0] + bE * (hz - hz_jm) * sxz - bE * (hy - hy_km) * sxy,
                         aE * E[1] + bE * (hx - hx_km) * syx - bE * (hz - hz_im) * syz,
                         aE * E[2] + bE * (hy - hy_im) * szy - bE * (hx - hx_jm) * szx};
@@ -308,7 +310,7 @@ __device__ __forceinline__ void e_cell(const Dev<R> &d, unsigned parts, unsigned
         }
     }
     if (gain && (gparts & (E_GAINE | E_GAINPOP))) {
-        const R coef = coef_bE(d, id) / d.dt;
+        const R coef = coef_bE<SEL>(d, id) / d.dt;
         double dot = 0.0;
         for (int c = 0; c < 3; c++) {
             const long long o = (long long)c * d.ngain + gs;
@@ -784,10 +786,11 @@ __device__ __forceinline__ const Dev<R> &instance(const Dev<R> &d0, const Dev<R>
 // ---------------------------------------------------------------------------
 // Waves per SIMD the merged kernels are built for: 8 in FP32, where the
 // register allocator keeps them without memory spills (scalar values that do
-// not fit the 8-wave budget go to vector-register lanes); FP64 left to the
-// allocator (its merged form runs 5 waves per SIMD).
-template <class R>
-constexpr int kFullWaves = sizeof(R) == 4 ? 8 : 1;
+// not fit the 8-wave budget go to vector-register lanes); FP64 and the
+// batched form (whose instance views are read from device memory, where
+// every reload is a scalar-cache round trip) left to the allocator.
+template <class R, bool B>
+constexpr int kFullWaves = sizeof(R) == 4 && !B ? 8 : 1;
 
 // One tile of a plane: its origin, and what the host found about it for the
 // half-step the table belongs to (so the kernel branches on one loaded word
@@ -851,7 +854,7 @@ __device__ __forceinline__ unsigned tile_pml_mask(const Dev<R> &d, const MTile &
 }
 
 // one merged H tile-plane
-template <class R, bool CS, bool NT>
+template <class R, bool CS, bool NT, bool SEL>
 __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Box &L, const LeanStretch<R> &c)
 {
     const int i = t.i, j = t.j, k = t.k;
@@ -868,12 +871,12 @@ __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Bo
     const R ey_k1 = d.F[1][pk1], ey_i1 = d.F[1][pi1], ez_i1 = d.F[2][pi1];
     R hx = ld_once<NT>(&d.F[3][p]), hy = ld_once<NT>(&d.F[4][p]), hz = ld_once<NT>(&d.F[5][p]);
     if (tile_inside(L, t)) {
-        const R aH = coef_aH(d, id), bH = coef_bH(d, id);
+        const R aH = coef_aH<SEL>(d, id), bH = coef_bH<SEL>(d, id);
         hx = aH * hx - bH * (ez_j1 - ez) * lean_st<CS>(d, c, S_cHxz, j) + bH * (ey_k1 - ey) * lean_st<CS>(d, c, S_cHxy, k);
         hy = aH * hy - bH * (ex_k1 - ex) * lean_st<CS>(d, c, S_cHyx, k) + bH * (ez_i1 - ez) * lean_st<CS>(d, c, S_cHyz, i);
         hz = aH * hz + bH * (ex_j1 - ex) * lean_st<CS>(d, c, S_cHzx, j) - bH * (ey_i1 - ey) * lean_st<CS>(d, c, S_cHzy, i);
     } else {
-        h_cell<R, CS, NT>(d, true, d.has_pml ? tile_pml_mask(d, t, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
+        h_cell<R, CS, NT, SEL>(d, true, d.has_pml ? tile_pml_mask(d, t, false) : 0u, i, j, k, p, ex, ey, ez, ex_j1, ez_j1,
                       ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz, id, &c);
     }
     st_once<NT>(&d.F[3][p], hx);
@@ -882,16 +885,16 @@ __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Bo
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R>))) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R, B>))) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance_z<R, B>(d0, devs);
     MTile t;
     mtile_at(g, blockIdx.x, g.k_lo + (int)blockIdx.y, t);
-    h_tile<R, CS, NT>(d, t, L, c);
+    h_tile<R, CS, NT, !B>(d, t, L, c);
 }
 
 // one merged E tile-plane
-template <class R, bool CS, bool NT>
+template <class R, bool CS, bool NT, bool SEL>
 __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Box &L, const LeanStretch<R> &c)
 {
     const int i = t.i, j = t.j, k = t.k;
@@ -914,14 +917,14 @@ __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Bo
     // issuing their psi loads).
     bool keep;
     if (tile_inside(L, t)) {
-        const R aE = coef_aE(d, id), bE = coef_bE(d, id);
+        const R aE = coef_aE<SEL>(d, id), bE = coef_bE<SEL>(d, id);
         E[0] = aE * E[0] + bE * (hz - hz_jm) * lean_st<CS>(d, c, S_cExz, j) - bE * (hy - hy_km) * lean_st<CS>(d, c, S_cExy, k);
         E[1] = aE * E[1] + bE * (hx - hx_km) * lean_st<CS>(d, c, S_cEyx, k) - bE * (hz - hz_im) * lean_st<CS>(d, c, S_cEyz, i);
         E[2] = aE * E[2] + bE * (hy - hy_im) * lean_st<CS>(d, c, S_cEzy, i) - bE * (hx - hx_jm) * lean_st<CS>(d, c, S_cEzx, j);
         for (int q = 0; q < 3; q++) E[q] = id == 4 ? (R)0 : E[q];
         keep = special_id(d, id) | (id == 2);  // ID 2 without the ADE module is left as is
     } else {
-        e_cell<R, CS, NT>(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, t, true) : 0u, i, j, k, p, id, hx, hy, hz,
+        e_cell<R, CS, NT, SEL>(d, E_BASIC | E_PML, d.has_pml ? tile_pml_mask(d, t, true) : 0u, i, j, k, p, id, hx, hy, hz,
                       hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, E, true, &c);
         keep = special_id(d, id);
     }
@@ -932,12 +935,12 @@ __device__ __forceinline__ void e_tile(const Dev<R> &d, const MTile &t, const Bo
 }
 
 template <class R, bool CS, bool NT, bool B = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R>))) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R, B>))) void k_full_e(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
     const Dev<R> &d = instance_z<R, B>(d0, devs);
     MTile t;
     mtile_at(g, blockIdx.x, g.k_lo + (int)blockIdx.y, t);
-    e_tile<R, CS, NT>(d, t, L, c);
+    e_tile<R, CS, NT, !B>(d, t, L, c);
 }
 
 // Drude-Lorentz and gain cells, all of them: device cell index, with bit
@@ -1365,8 +1368,8 @@ __global__ __launch_bounds__(256) void k_pack_h(const PackMember<R> *pm, const u
     if (nstep[s] + q >= P.nt) return;
     MTile t;
     mtile_at(P.mh, (unsigned)(w & 0xffffffu), (int)((w >> kPackPlaneShift) & 0xffffffu), t);
-    if (P.csH) h_tile<R, true, NT>(P.d, t, P.Lh, P.sh);
-    else h_tile<R, false, NT>(P.d, t, P.Lh, P.sh);
+    if (P.csH) h_tile<R, true, NT, false>(P.d, t, P.Lh, P.sh);
+    else h_tile<R, false, NT, false>(P.d, t, P.Lh, P.sh);
 }
 
 template <class R, bool NT>
@@ -1379,8 +1382,8 @@ __global__ __launch_bounds__(256) void k_pack_e(const PackMember<R> *pm, const u
     if (nstep[s] + q >= P.nt) return;
     MTile t;
     mtile_at(P.me, (unsigned)(w & 0xffffffu), (int)((w >> kPackPlaneShift) & 0xffffffu), t);
-    if (P.csE) e_tile<R, true, NT>(P.d, t, P.Le, P.se);
-    else e_tile<R, false, NT>(P.d, t, P.Le, P.se);
+    if (P.csE) e_tile<R, true, NT, false>(P.d, t, P.Le, P.se);
+    else e_tile<R, false, NT, false>(P.d, t, P.Le, P.se);
 }
 
 // TF/SF corrections of every member (h: the H half-step's)
