@@ -888,7 +888,7 @@ public:
         const int nb = dft_pending;
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
-            CFD_LAUNCH((k_dft<R>), dim3(port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
+            CFD_LAUNCH((k_dft<R>), dim3(4 * port_blocks), dim3(256), 0, stream, port_tab, (int)dports.size(), nb,
                                nbase_cur, dft_first, m.dft_stride, twE, twH, (int)m.freqs.size(), m.nt);
         });
         HIP_OK(hipGetLastError());
@@ -1412,7 +1412,7 @@ public:
         const int nbatch = dft_pending;
         dft_pending = 0;
         timer.run(stream, "port_dft", [&] {
-            CFD_LAUNCH((k_dft<R, true>), dim3(a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, ports_all,
+            CFD_LAUNCH((k_dft<R, true>), dim3(4 * a.port_blocks, (unsigned)sims.size()), dim3(256), 0, stream, ports_all,
                                nports, nbatch, nullptr, dft_first, a.m.dft_stride, a.twE, a.twH, (int)a.m.freqs.size(), a.m.nt);
         });
         HIP_OK(hipGetLastError());
@@ -1611,7 +1611,7 @@ public:
                 CFD_LAUNCH(k_pack_port<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
             });
             if (any_dft) timer.run(stream, "pack_dft", [&] {
-                CFD_LAUNCH(k_pack_dft<R>, dim3(nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
+                CFD_LAUNCH(k_pack_dft<R>, dim3(4 * nb_port), blk, 0, stream, dmem, st_port, nm, nstep, q);
             });
         }
         HIP_OK(hipGetLastError());

@@ -1267,32 +1267,47 @@ __global__ __launch_bounds__(256) void k_post_e(Dev<R> d, PostE<R> P, const int 
 // accumulator is read once, gets the batch's terms in step order (the same
 // additions, in the same order, as accumulating every sampled step), and is
 // written once.  E samples carry phase 2 pi f (n+1) dt, H samples (n+1/2) dt.
+// A workgroup takes 64 points, the quarter `quarter` of port block blk0 (the
+// port sampling's 256-point blocks): their samples are staged in LDS once,
+// and the four wavefronts split the frequencies (lane = point), so the
+// samples cross HBM once instead of once per frequency.  Every thread of the
+// workgroup calls it (it synchronises).
 template <class R>
-__device__ __forceinline__ void dft_block(const PortDesc<R> *ports, int nports, int blk0, int nb, int n0, int stride,
-                                          const double *twE, const double *twH, int nf, int ntot)
+__device__ __forceinline__ void dft_block(const PortDesc<R> *ports, int nports, int blk0, int quarter, int nb, int n0,
+                                          int stride, const double *twE, const double *twH, int nf, int ntot)
 {
+    __shared__ R xs[16 * 6][64];  // [sampled step * 6 + component][point]; nb <= 16
     int q = 0;
     while (q + 1 < nports && blk0 >= ports[q + 1].bstart) q++;
     const PortDesc<R> P = ports[q];
-    const int t = (blk0 - P.bstart) * 256 + threadIdx.x;
-    if (t >= P.npts) return;
     const long long np_ = P.npts;
-    for (int f = 0; f < nf; f++) {
-        double *acc = P.dft + (long long)f * 12 * np_ + t;
-        double a[12];
-        for (int w = 0; w < 12; w++) a[w] = acc[w * np_];
-        for (int b = 0; b < nb; b++) {
-            const long long n = n0 + (long long)b * stride;
-            const double *wE = twE + ((long long)f * ntot + n) * 2, *wH = twH + ((long long)f * ntot + n) * 2;
-            const R *x = P.batch + (long long)b * 6 * np_ + t;
-            for (int c = 0; c < 6; c++) {
-                const double *w = c < 3 ? wE : wH;
-                const double v = (double)x[c * np_];
-                a[2 * c] = a[2 * c] + v * w[0];
-                a[2 * c + 1] = a[2 * c + 1] - v * w[1];
-            }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    {
+        const int t0 = (blk0 - P.bstart) * 256 + quarter * 64;
+        if (t0 >= P.npts) return;  // the same for the whole workgroup
+        for (int e = threadIdx.x; e < nb * 6 * 64; e += 256) {
+            const int row = e >> 6, pt = e & 63;
+            xs[row][pt] = t0 + pt < P.npts ? P.batch[(long long)row * np_ + t0 + pt] : (R)0;
         }
-        for (int w = 0; w < 12; w++) acc[w * np_] = a[w];
+        __syncthreads();
+        const int t = t0 + lane;
+        if (t < P.npts)
+            for (int f = wave; f < nf; f += 4) {
+                double *acc = P.dft + (long long)f * 12 * np_ + t;
+                double a[12];
+                for (int w = 0; w < 12; w++) a[w] = acc[w * np_];
+                for (int b = 0; b < nb; b++) {
+                    const long long n = n0 + (long long)b * stride;
+                    const double *wE = twE + ((long long)f * ntot + n) * 2, *wH = twH + ((long long)f * ntot + n) * 2;
+                    for (int c = 0; c < 6; c++) {
+                        const double *w = c < 3 ? wE : wH;
+                        const double v = (double)xs[b * 6 + c][lane];
+                        a[2 * c] = a[2 * c] + v * w[0];
+                        a[2 * c + 1] = a[2 * c + 1] - v * w[1];
+                    }
+                }
+                for (int w = 0; w < 12; w++) acc[w * np_] = a[w];
+            }
     }
 }
 template <class R, bool B = false>
@@ -1301,7 +1316,7 @@ __global__ __launch_bounds__(256) void k_dft(const PortDesc<R> *ports, int nport
 {
     if (nbase) n0 += *nbase;
     if (B) ports += (long long)blockIdx.y * nports;  // the instance's port table
-    dft_block(ports, nports, (int)blockIdx.x, nb, n0, stride, twE, twH, nf, ntot);
+    dft_block(ports, nports, (int)blockIdx.x >> 2, (int)blockIdx.x & 3, nb, n0, stride, twE, twH, nf, ntot);
 }
 
 // ---------------------------------------------------------------------------
@@ -1459,15 +1474,16 @@ template <class R>
 __global__ __launch_bounds__(256) void k_pack_dft(const PackMember<R> *pm, const int *start, int nmem,
                                                   const int *nstep, int q)
 {
-    const int s = pack_member(start, nmem, blockIdx.x);
+    const int blk = (int)blockIdx.x >> 2;  // four workgroups per port block
+    const int s = pack_member(start, nmem, blk);
     const PackMember<R> &P = pm[s];
     const int n = nstep[s] + q;
     if (n >= P.nt || !P.nf || n % P.dft_stride) return;
     const int slot = (n / P.dft_stride) % 16;
     const int last = (P.nt - 1) / P.dft_stride * P.dft_stride;
     if (slot != 15 && n != last) return;
-    dft_block(P.ports, P.nports, (int)blockIdx.x - start[s], slot + 1, n - slot * P.dft_stride, P.dft_stride, P.twE,
-              P.twH, P.nf, P.nt);
+    dft_block(P.ports, P.nports, blk - start[s], (int)blockIdx.x & 3, slot + 1, n - slot * P.dft_stride, P.dft_stride,
+              P.twE, P.twH, P.nf, P.nt);
 }
 
 // every member's step base moves on by the length of the launched sequence
