@@ -627,7 +627,8 @@ public:
     int spoles[2] = {-1, -1};  // special pass instantiation (launch_special)
     // the passes after the E update in one launch (k_post_e), when their
     // cells allow it (post_fusable)
-    bool post_fused = false, post_ports = false;  // post_ports: the port sampling is in it
+    // post_srcs / post_ports: the sources / the port sampling are in it
+    bool post_fused = false, post_srcs = false, post_ports = false;
     PostE<R> post{};
     int post_blocks = 0;
     bool nt_fields = false;  // streamed own-cell field traffic (see stream_fields)
@@ -914,12 +915,13 @@ public:
         sources(n);
         if (!dports.empty()) ports(n);
     }
-    // special cells, TF/SF-E, sources and (post_ports) the port sampling of
-    // step n in one launch (k_post_e), with the port bookkeeping of ports();
-    // else the port sampling follows as its own launch
+    // special cells, TF/SF-E, (post_srcs) sources and (post_ports) the port
+    // sampling of step n in one launch (k_post_e), with the port bookkeeping
+    // of ports(); the passes left out follow as their own launches, in the
+    // reference's order
     void post_e(int n) {
         if (post_ports && (n < 0 || n >= m.nt)) throw Error("port time index out of range");
-        if (nsgroup && !nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
+        if (post_srcs && nsgroup && !nbase_cur && (n < 0 || n > m.nt)) throw Error("source time index out of range");
         PostE<R> P = post;
         P.slot = -1;
         if (post_ports && !m.freqs.empty() && n % m.dft_stride == 0) {
@@ -936,6 +938,7 @@ public:
 #undef CFD_POST
         });
         HIP_OK(hipGetLastError());
+        if (!post_srcs) sources(n);
         if (!post_ports) {
             if (!dports.empty()) ports(n);
             return;
@@ -945,7 +948,7 @@ public:
     // the post-E launch's tables and workgroup ranges, when post_fusable
     // allows it (CFD_FUSE_POST=0: separate launches, for A/B runs)
     void setup_post() {
-        post_fused = post_ports = false;
+        post_fused = post_srcs = post_ports = false;
         if (const char *v = std::getenv("CFD_FUSE_POST"))
             if (std::string(v) == "0") return;
         std::vector<long long> te, sp;
@@ -960,9 +963,10 @@ public:
             pcs.emplace_back();
             for (size_t q = 0; q < P.ijk.size(); q += 3) pcs.back().push_back(dev_index(P.ijk[q], P.ijk[q + 1], P.ijk[q + 2]));
         }
-        const int level = post_fusable(te, sp, pcs);
-        if (!level) return;
-        post_ports = level == 2 && !dports.empty();
+        const int parts = post_fusable(te, sp, pcs);
+        if (!(parts & 1)) return;
+        post_srcs = (parts & 2) != 0;
+        post_ports = (parts & 4) && !dports.empty();
         PostE<R> P{};
         P.special = special;
         P.nspecial = form == kFormGrid ? 0 : nspecial;
@@ -975,8 +979,8 @@ public:
         if constexpr (std::is_same<R, float>::value) { P.terms32 = tterm32; P.tab32 = hinc32; }
         P.tablen = tablen;
         P.grp = sgroup;
-        P.ngroups = nsgroup;
-        P.b3 = P.b2 + (nsgroup + 255) / 256;
+        P.ngroups = post_srcs ? nsgroup : 0;
+        P.b3 = P.b2 + (P.ngroups + 255) / 256;
         P.sidx = sidx;
         P.swave = swave;
         P.sstride = m.nt + 1;
@@ -984,13 +988,18 @@ public:
         P.nports = post_ports ? (int)dports.size() : 0;
         post = P;
         post_blocks = P.b3 + (post_ports ? port_blocks : 0);
-        post_fused = post_blocks > 0;
-        if (!post_fused) post_ports = false;
+        // one launch only where it replaces two or more: a pass alone keeps
+        // its own kernel (and register budget)
+        const int roles = (P.nspecial > 0) + (P.ntg > 0) + (P.ngroups > 0) + (post_ports ? 1 : 0);
+        post_fused = roles >= 2;
+        if (!post_fused) post_srcs = post_ports = false;
     }
-    // May the post-E passes run side by side?  The E values they write must
-    // lie at cells that no other of them writes (else 0); the port sampling
-    // joins them (2) only if it samples none of those cells, else it stays a
-    // launch of its own after them (1).  Cell-level test: conservative.
+    // Which post-E passes may run side by side (bit mask; cell-level test,
+    // conservative)?  Bit 0: the special cells and the TF/SF-E targets are
+    // distinct cells (else no fused launch); bit 1: the sources' cells are
+    // neither, so they join (else they follow as their own launch); bit 2:
+    // the sources joined and the ports sample none of those cells, so the
+    // port sampling joins too (else it follows as its own launch).
     // te: TF/SF-E target cells; sp: special cells (flag bits included)
     int post_fusable(std::vector<long long> te, const std::vector<long long> &sp,
                      const std::vector<std::vector<long long>> &port_cells) const {
@@ -1012,8 +1021,9 @@ public:
             }
             return false;
         };
-        if (meet(a, te) || meet(a, c) || meet(te, c)) return 0;
-        return meet(pc, a) || meet(pc, te) || meet(pc, c) ? 1 : 2;
+        if (meet(a, te)) return 0;
+        if (meet(c, a) || meet(c, te)) return 1;
+        return meet(pc, a) || meet(pc, te) || meet(pc, c) ? 3 : 7;
     }
 
     // Step blocks replayed from a HIP graph: a block of graph_block() steps
