@@ -570,7 +570,8 @@ public:
     }
     // Tiles of a merged half-step (see MGrid): 32 x 8 tiles in the x-edge
     // columns that hold the x CPML slabs, 64 x 4 tiles between them; one
-    // plane's tile origins, x fastest within each of the two kinds.
+    // plane's tile origins, the 64 x 4 tiles first (ordered as below), then
+    // the edge tiles, x fastest.
     // CFD_TILE_EDGE32=0 gives 64-wide tiles everywhere (A/B runs).
     // y_org: the first tile row (the E pass starts at its first updated row,
     // j = 1 without y periodicity: row 0 would idle a quarter of the first
@@ -595,9 +596,17 @@ public:
             if (x0 >= L.i0 && x0 + tw <= L.i1 && y0 >= L.j0 && y0 + th <= L.j1) f |= kTileInnerXY;
             return MTileEntry{(unsigned)x0 | (unsigned)y0 << 15 | (narrow ? 1u << 31 : 0u), f};
         };
+        // 64 x 4 tiles by groups of eight tile rows, each group's tiles x
+        // major: workgroups go to the XCDs in turn, so every tile of a row
+        // then runs on the same XCD and finds the neighbour line of the tile
+        // beside it (i -+ 1) in that XCD's L2 (256^3 FP32: both merged kernels
+        // 1.3 % faster); with a multiple of 8 tiles per plane the tile of the
+        // plane below (k -+ 1) runs there too
         std::vector<MTileEntry> t;
-        for (int y0 = y_org; y0 < m.Ny; y0 += 4)
-            for (int b = 0; b < nw; b++) t.push_back(entry(xa + 64 * b, y0, false));
+        const int nrow = (m.Ny - y_org + 3) / 4;
+        for (int g = 0; g < nrow; g += 8)
+            for (int b = 0; b < nw; b++)
+                for (int r = g; r < g + 8 && r < nrow; r++) t.push_back(entry(xa + 64 * b, y_org + 4 * r, false));
         for (int y0 = y_org; y0 < m.Ny; y0 += 8) {
             for (int x0 = 0; x0 < xa; x0 += 32) t.push_back(entry(x0, y0, true));
             for (int x0 = xb; x0 < m.Nx; x0 += 32) t.push_back(entry(x0, y0, true));
