@@ -265,8 +265,9 @@ def cpu_baseline(scale_steps=200):
         return {"value": round(r["mcells_per_s"], 3), "unit": "Mcell-updates/s", "cores": r["threads"],
                 "kind": "reference",
                 "sample": "%d steps of the same 256^3 grid (CPML + TF/SF; the reference has no 3-D Drude-Lorentz, its "
-                          "ID-2 cells are skipped), FP64, reference sources at -O2 -fopenmp, %.1f s" %
-                          (scale_steps, r["seconds"])}
+                          "ID-2 cells are skipped), FP64, reference sources at -O2 -fopenmp, %.1f s; the reference "
+                          "runs its own CPML-H sign and TF/SF face forms, the GPU the fixed ones (same work per "
+                          "cell: a rate comparison)" % (scale_steps, r["seconds"])}
     d = tempfile.mkdtemp(prefix="bench_port_")
     case = configs.c2(d, nt=4, nfreq=0)
     case.ports = []
