@@ -327,3 +327,50 @@ def test_sources_sharing_a_cell_f64_bit_exact(tmp_path):
     e, o = run_both(case, 300)
     assert_fields_equal(e, o, "shared source cell")
     assert np.abs(o.field(2)).max() > 0
+
+
+def _post_case(directory, src, port_z, sphere_x):
+    """40 x 36 x 32 cells, 5-cell CPML, a normal-incidence TF/SF box 8 cells in
+    from the faces, a Drude-Lorentz sphere of radius 4 cells at (sphere_x, 18,
+    16), a soft Ez source at src and a plane port across the grid at
+    port_z."""
+    from cpsfdtd_amd.case import CaseWriter, Shape, dlambda_for_fwhm
+    d = 10e-9
+    n = (40, 36, 32)
+    w = CaseWriter(directory)
+    w.materials([(1, 1, 0, 0), (1, 1, 0, 0), (1, 1, 0, 0)])
+    w.grid(n, (d, d, d), [Shape(2, "ellipsoid", x0=sphere_x * d, y0=18 * d, z0=16 * d, a=4 * d, b=4 * d, c=4 * d)])
+    w.cpml(5, m=3.0, malp=1.0, kapmax=5.0, sig_ratio=1.0, alpmax=0.05)
+    w.tfsf(500e-9, dlambda_for_fwhm(500e-9, 5e-15), 1e-3, 0.0, 20 * d, 18 * d, 0.0, 0.0, 0.0, 1.0, 90.0,
+           ((8, 8), (8, 8), (8, 8)), 60)
+    w.drude_lorentz(3.7, [(1.37e16, 2.7e13)], [(4.2e15, 1e15, 1.0), (6.1e15, 2e15, 0.5)])
+    w.source(500e-9, 5e-15, 10e-15)
+    w.point_source(*src, comp=2, amp=2.0)
+    w.plane_port_cells("pp.txt", "z", port_z, 2, n[0] - 3, 2, n[1] - 3, (d, d, d))
+    return w.run(60, record_ports=True, flags=FIXED)
+
+
+@pytest.mark.parametrize("kind", ["all", "ports_apart", "sources_apart", "none"])
+def test_post_e_launch_f64_bit_exact(tmp_path, kind):
+    """The passes after the E update (special cells, TF/SF-E, sources, port
+    sampling) share one launch as far as their cells allow: every outcome of
+    the host's test, with the launches it is expected to choose, equals the
+    oracle bit for bit.  all: nothing shared; ports_apart: the port plane
+    crosses the TF/SF x and y faces; sources_apart: the source sits in the
+    Drude-Lorentz sphere; none: the sphere crosses a TF/SF face."""
+    src, port_z, sphere_x = {"all": ((12, 12, 12), 4, 20), "ports_apart": ((12, 12, 12), 16, 20),
+                             "sources_apart": ((20, 18, 16), 4, 20), "none": ((12, 12, 12), 4, 8)}[kind]
+    case = _post_case(str(tmp_path), src, port_z, sphere_x)
+    e, o = run_both(case, 60)
+    assert_fields_equal(e, o, kind)
+    assert np.array_equal(e.port_samples(0), o.port_samples(0))
+    assert np.abs(o.field(2)).max() > 0
+    t = eng.Engine(case, eng.CFD_F64)
+    t.set_timing(True)
+    t.step(3)
+    names = set(t.kernel_times())
+    t.close()
+    expect = {"all": {"post_e"}, "ports_apart": {"post_e", "ports"}, "sources_apart": {"post_e", "sources", "ports"},
+              "none": {"special_e", "tfsf_e", "sources", "ports"}}[kind]
+    launched = names & {"post_e", "special_e", "tfsf_e", "sources", "ports"}
+    assert launched == expect, (kind, sorted(names))
