@@ -1180,17 +1180,27 @@ public:
             keep_pool_memory();
             HIP_OK(hipMallocAsync(&buf, (size_t)chunk * rows * sizeof(double), stream));
         }
-        for (long long t0 = 0; t0 < np_; t0 += chunk) {
-            const int n = (int)std::min(chunk, np_ - t0);
-            CFD_LAUNCH(k_dft_transpose, dim3((unsigned)((n + 31) / 32), (unsigned)((rows + 31) / 32)), dim3(256), 0, stream,
-                       p.dft, np_, rows, t0, n, buf);
-            HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpyAsync(host + t0 * rows, buf, (size_t)n * rows * sizeof(double), hipMemcpyDeviceToHost, stream));
+        auto release = [&] {
+            (void)hipStreamSynchronize(stream);
+            std::lock_guard<std::mutex> lk(capture_mutex());
+            (void)hipFreeAsync(buf, stream);
+            (void)hipStreamSynchronize(stream);
+        };
+        try {
+            for (long long t0 = 0; t0 < np_; t0 += chunk) {
+                const int n = (int)std::min(chunk, np_ - t0);
+                CFD_LAUNCH(k_dft_transpose, dim3((unsigned)((n + 31) / 32), (unsigned)((rows + 31) / 32)), dim3(256), 0,
+                           stream, p.dft, np_, rows, t0, n, buf);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipMemcpyAsync(host + t0 * rows, buf, (size_t)n * rows * sizeof(double), hipMemcpyDeviceToHost,
+                                      stream));
+            }
+            HIP_OK(hipStreamSynchronize(stream));
+        } catch (...) {
+            release();
+            throw;
         }
-        HIP_OK(hipStreamSynchronize(stream));
-        std::lock_guard<std::mutex> lk(capture_mutex());
-        HIP_OK(hipFreeAsync(buf, stream));
-        HIP_OK(hipStreamSynchronize(stream));
+        release();
     }
     void port_intensity(int q, double *host) override {
         const PortDesc<R> &p = dports.at(q);
