@@ -14,6 +14,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cpsfdtd.h"
@@ -112,10 +113,36 @@ inline std::mutex &capture_mutex() {
     return m;
 }
 
-// FNV-1a over raw bytes: a batch checks that its instances share a geometry
+// Two pinned host staging buffers of kStageBytes each, allocated once per
+// process (hipHostMalloc) and shared by every simulation's read-backs under
+// stage_mutex(); pageable destinations copy from device memory at a few GB/s,
+// pinned ones at the link's rate.
+constexpr size_t kStageBytes = size_t(32) << 20;
+inline std::mutex &stage_mutex() {
+    static std::mutex m;
+    return m;
+}
+inline double **pinned_stage() {  // call with stage_mutex() held
+    static double *buf[2] = {nullptr, nullptr};
+    for (double *&b : buf)
+        if (!b && hipHostMalloc((void **)&b, kStageBytes, hipHostMallocDefault) != hipSuccess) {
+            b = nullptr;
+            throw Error("cannot allocate pinned staging memory");
+        }
+    return buf;
+}
+
+// FNV-1a-style signature over raw bytes, eight at a time (the tail byte by
+// byte): a batch checks that its instances share a geometry
 inline void sig_add(uint64_t &h, const void *p, size_t n) {
     const unsigned char *b = static_cast<const unsigned char *>(p);
-    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h = (h ^ w) * 1099511628211ull;
+    }
+    for (; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
 }
 template <class T>
 inline void sig_add(uint64_t &h, const std::vector<T> &v) {
@@ -370,12 +397,25 @@ public:
         const size_t ncell = (size_t)d.slab * (m.Nz + 1);
         for (int c = 0; c < 6; c++) d.F[c] = alloc<R>(ncell);
         nt_fields = stream_fields(6 * ncell * sizeof(R));
-        // material IDs in device order
+        // material IDs in device order: the reference's [i][j][k] order (k
+        // fastest) transposed in 16 x 16 (i, k) blocks, planes of k blocks
+        // shared among host threads
         {
             std::vector<uint8_t> id(ncell, 0);
-            for (int i = 0; i <= m.Nx; i++)
-                for (int j = 0; j <= m.Ny; j++)
-                    for (int k = 0; k <= m.Nz; k++) id[dev_index(i, j, k)] = (uint8_t)m.id[m.flat(i, j, k)];
+            const int nk = m.Nz + 1, nkb = (nk + 15) / 16;
+            const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            auto part = [&](int t) {
+                for (int kb = t; kb < nkb; kb += nth)
+                    for (int j = 0; j <= m.Ny; j++)
+                        for (int i0 = 0; i0 <= m.Nx; i0 += 16)
+                            for (int i = i0; i <= std::min(m.Nx, i0 + 15); i++)
+                                for (int k = 16 * kb; k < std::min(nk, 16 * kb + 16); k++)
+                                    id[dev_index(i, j, k)] = (uint8_t)m.id[m.flat(i, j, k)];
+            };
+            std::vector<std::thread> pool;
+            for (int t = 1; t < nth; t++) pool.emplace_back(part, t);
+            part(0);
+            for (std::thread &th : pool) th.join();
             d.id = upload<uint8_t>(id);
         }
         d.aE = upload<R>(m.aE); d.bE = upload<R>(m.bE); d.aH = upload<R>(m.aH); d.bH = upload<R>(m.bH);
@@ -420,8 +460,10 @@ public:
         }
         // Drude-Lorentz and gain cells: slots numbered in device order (x fastest),
         // so the lanes of a wavefront touch consecutive state words
+        // (the map is -1 elsewhere: filled and scattered on the device)
         if (m.has_dl || m.has_gain) {
-            std::vector<int> aux(ncell, -1);
+            int *aux = alloc<int>(ncell);
+            HIP_OK(hipMemsetAsync(aux, 0xff, ncell * sizeof(int), stream));
             auto assign = [&](const std::vector<long long> &cells, std::vector<int> &slot_of) {
                 std::vector<std::pair<long long, int>> order;
                 order.reserve(cells.size());
@@ -433,14 +475,20 @@ public:
                 }
                 std::sort(order.begin(), order.end());
                 slot_of.assign(cells.size(), 0);
+                std::vector<long long> at(order.size());
                 for (size_t s2 = 0; s2 < order.size(); s2++) {
-                    aux[order[s2].first] = (int)s2;
+                    at[s2] = order[s2].first;
                     slot_of[order[s2].second] = (int)s2;
                 }
+                if (at.empty()) return;
+                long long *dat = upload<long long>(at);
+                CFD_LAUNCH(k_slot_scatter, dim3((unsigned)((at.size() + 255) / 256)), dim3(256), 0, stream, aux, dat,
+                           (long long)at.size());
+                HIP_OK(hipGetLastError());
             };
             assign(m.dl_cells, dl_slot);
             assign(m.gain_cells, gain_slot);
-            d.aux = upload<int>(aux);
+            d.aux = aux;
         }
         d.dt = (R)m.dt;
         if (m.has_dl) {
@@ -1167,35 +1215,53 @@ public:
         const PortDesc<R> &p = dports.at(q);
         if (!p.dft) throw Error("no DFT frequencies configured");
         dft_flush();
-        // device [f][c][re,im][point] -> host [point][f][c][re,im], transposed
-        // on the device in chunks of points (at most 32 MiB each) and copied
-        // straight into host
+        // device [f][c][re,im][point] -> host [point][f][c][re,im]: transposed
+        // on the device in chunks of points (32 MiB each), each copied to one
+        // of two pinned staging buffers while the host copies the previous
+        // chunk out of the other
         const int rows = (int)m.freqs.size() * 12;
         const long long np_ = p.npts;
         if (!np_) return;
-        const long long chunk = std::min(np_, std::max(32LL, (4LL << 20) / rows / 32 * 32));
+        const long long chunk = std::min(np_, std::max(32LL, (long long)(kStageBytes / 8) / rows / 32 * 32));
+        const size_t cbytes = (size_t)chunk * rows * sizeof(double);
+        std::lock_guard<std::mutex> stage_lk(stage_mutex());
+        double **pin = pinned_stage();
         double *buf = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
         {
             std::lock_guard<std::mutex> lk(capture_mutex());
             keep_pool_memory();
-            HIP_OK(hipMallocAsync(&buf, (size_t)chunk * rows * sizeof(double), stream));
+            HIP_OK(hipMallocAsync(&buf, 2 * cbytes, stream));
         }
         auto release = [&] {
             (void)hipStreamSynchronize(stream);
+            for (hipEvent_t &e : ev)
+                if (e) (void)hipEventDestroy(e);
             std::lock_guard<std::mutex> lk(capture_mutex());
             (void)hipFreeAsync(buf, stream);
             (void)hipStreamSynchronize(stream);
         };
         try {
-            for (long long t0 = 0; t0 < np_; t0 += chunk) {
+            for (hipEvent_t &e : ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            const long long nch = (np_ + chunk - 1) / chunk;
+            auto copy_out = [&](long long c) {  // chunk c: staged -> host
+                const long long t0 = c * chunk;
                 const int n = (int)std::min(chunk, np_ - t0);
+                HIP_OK(hipEventSynchronize(ev[c & 1]));
+                std::memcpy(host + t0 * rows, pin[c & 1], (size_t)n * rows * sizeof(double));
+            };
+            for (long long c = 0; c < nch; c++) {
+                const long long t0 = c * chunk;
+                const int n = (int)std::min(chunk, np_ - t0);
+                double *db = buf + (c & 1) * (size_t)chunk * rows;
                 CFD_LAUNCH(k_dft_transpose, dim3((unsigned)((n + 31) / 32), (unsigned)((rows + 31) / 32)), dim3(256), 0,
-                           stream, p.dft, np_, rows, t0, n, buf);
+                           stream, p.dft, np_, rows, t0, n, db);
                 HIP_OK(hipGetLastError());
-                HIP_OK(hipMemcpyAsync(host + t0 * rows, buf, (size_t)n * rows * sizeof(double), hipMemcpyDeviceToHost,
-                                      stream));
+                HIP_OK(hipMemcpyAsync(pin[c & 1], db, (size_t)n * rows * sizeof(double), hipMemcpyDeviceToHost, stream));
+                HIP_OK(hipEventRecord(ev[c & 1], stream));
+                if (c > 0) copy_out(c - 1);
             }
-            HIP_OK(hipStreamSynchronize(stream));
+            copy_out(nch - 1);
         } catch (...) {
             release();
             throw;

@@ -1094,6 +1094,14 @@ __global__ __launch_bounds__(256) void k_nonfinite(Dev<R> d, long long n, int *c
     }
 }
 
+// slot map of the special cells: aux[cell[s]] = s (the rest of the map holds
+// -1 from a fill)
+__global__ __launch_bounds__(256) void k_slot_scatter(int *aux, const long long *cell, long long n)
+{
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (s < n) aux[cell[s]] = (int)s;
+}
+
 // one device int: the block start of a replayed step graph, or a counter's
 // zero (a kernel, not a memset, so it may sit inside a captured graph)
 __global__ void k_set_int(int *dst, int v)
