@@ -403,19 +403,14 @@ public:
         {
             std::vector<uint8_t> id(ncell, 0);
             const int nk = m.Nz + 1, nkb = (nk + 15) / 16;
-            const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-            auto part = [&](int t) {
+            host_parallel([&](int t, int nth) {
                 for (int kb = t; kb < nkb; kb += nth)
                     for (int j = 0; j <= m.Ny; j++)
                         for (int i0 = 0; i0 <= m.Nx; i0 += 16)
                             for (int i = i0; i <= std::min(m.Nx, i0 + 15); i++)
                                 for (int k = 16 * kb; k < std::min(nk, 16 * kb + 16); k++)
                                     id[dev_index(i, j, k)] = (uint8_t)m.id[m.flat(i, j, k)];
-            };
-            std::vector<std::thread> pool;
-            for (int t = 1; t < nth; t++) pool.emplace_back(part, t);
-            part(0);
-            for (std::thread &th : pool) th.join();
+            });
             d.id = upload<uint8_t>(id);
         }
         d.aE = upload<R>(m.aE); d.bE = upload<R>(m.bE); d.aH = upload<R>(m.aH); d.bH = upload<R>(m.bH);

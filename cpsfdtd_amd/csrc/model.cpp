@@ -206,8 +206,21 @@ void load_structure(Model &m, const char *matf, const char *meshf) {
         Painter p(m);
         for (const ShapeRec &s : shapes) p.paint(s, &shapes);
     }
-    for (int16_t v : m.id)
-        if (v < 0 || v >= m.NoMedia) throw Error("structure uses material " + std::to_string(v) + ", not defined in the material file");
+    {
+        const int none = -100000;  // no 16-bit ID has this value
+        std::vector<int> bad(8, none);  // per thread: an undefined material it found
+        const size_t n = m.id.size();
+        host_parallel([&](int t, int nth) {
+            const size_t a = n * t / nth, b = n * (t + 1) / nth;
+            for (size_t q = a; q < b; q++)
+                if (m.id[q] < 0 || m.id[q] >= m.NoMedia) {
+                    bad[t] = m.id[q];
+                    break;
+                }
+        });
+        for (int v : bad)
+            if (v != none) throw Error("structure uses material " + std::to_string(v) + ", not defined in the material file");
+    }
     // Structure.cpp:309-359
     const int len[kNumStretch] = {m.Nz, m.Nz, m.Ny, m.Ny, m.Nx, m.Nx, m.Nz, m.Nz, m.Ny, m.Ny, m.Nx, m.Nx};
     const double inv[kNumStretch] = {1.0 / m.dz, 1.0 / m.dz, 1.0 / m.dy, 1.0 / m.dy, 1.0 / m.dx, 1.0 / m.dx,
@@ -429,25 +442,39 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
     const int far = (m.flags & CFD_FIX_TFSF_FARFACE) ? 0 : 1;
 
     // targets in order of first appearance; terms kept flat with their
-    // target's number, grouped per target (stable) once every face is done
-    std::unordered_map<unsigned long long, int> eidx, hidx;  // key: cell * 8 + component
+    // target's number, grouped per target (stable) once every face is done.
+    // Face (a, e) corrects cells whose coordinate a is that face's plane (E:
+    // the box face; H: one cell outside the low face, `far` past the high
+    // one), each (cell, component) once.  A target can therefore come from a
+    // second face only if a second coordinate lies on its axis's planes: only
+    // those targets are looked up (key: cell * 8 + component); every other is
+    // new where it appears.
+    std::unordered_map<unsigned long long, int> eidx, hidx;
     std::vector<int> eown, hown;
     std::vector<TfsfTerm> eterms, hterms;
     {
         const size_t nrec = 2 * ((size_t)(i2 - i1 + 1) * (k2 - k1 + 1) + (size_t)(j2 - j1 + 1) * (k2 - k1 + 1) +
                                  (size_t)(i2 - i1 + 1) * (j2 - j1 + 1));
-        eidx.reserve(2 * nrec); hidx.reserve(2 * nrec);
+        m.tfE.reserve(2 * nrec); m.tfH.reserve(2 * nrec);
         eown.reserve(2 * nrec); hown.reserve(2 * nrec);
         eterms.reserve(2 * nrec); hterms.reserve(2 * nrec);
         m.tf_face_terms.reserve(4 * nrec);
     }
+    const int planeE[3][2] = {{i1, i2}, {j1, j2}, {k1, k2}};
+    const int planeH[3][2] = {{i1 - 1, i2 + far}, {j1 - 1, j2 + far}, {k1 - 1, k2 + far}};
     auto add = [&](bool h, int comp, int i, int j, int k, double coef, double delay, double prof) {
-        auto &idx = h ? hidx : eidx;
         auto &tg = h ? m.tfH : m.tfE;
-        const unsigned long long key = (unsigned long long)m.flat(i, j, k) * 8u + (unsigned)comp;
-        const auto ins = idx.emplace(key, (int)tg.size());
-        if (ins.second) tg.push_back({comp, i, j, k, 0, 0});
-        (h ? hown : eown).push_back(ins.first->second);
+        const int(&P)[3][2] = h ? planeH : planeE;
+        const int on = (i == P[0][0] || i == P[0][1]) + (j == P[1][0] || j == P[1][1]) + (k == P[2][0] || k == P[2][1]);
+        int t = (int)tg.size();
+        if (on >= 2) {
+            const unsigned long long key = (unsigned long long)m.flat(i, j, k) * 8u + (unsigned)comp;
+            const auto ins = (h ? hidx : eidx).emplace(key, t);
+            t = ins.first->second;
+            if (ins.second) tg.push_back({comp, i, j, k, 0, 0});
+        } else
+            tg.push_back({comp, i, j, k, 0, 0});
+        (h ? hown : eown).push_back(t);
         (h ? hterms : eterms).push_back({coef, delay, prof});
     };
     // a face alone: its records' corrections, each target once; the z1
@@ -481,26 +508,41 @@ void load_tfsf(Model &m, const char *path, int nt_req) {
                 if (!m.pby) for (int i = i1; i < i2; ++i) recs.push_back({i, j2, c0, 1});
                 if (!m.pbx) for (int j = j1; j < j2; ++j) recs.push_back({i2, j, c0, 2});
             }
-            for (const Rec &r : recs) {
+            // incident values of each record's component pairs: the H one (for
+            // the E correction) half a cell across the face, the E one on it;
+            // with CFD_FIX_TFSF_YEE both also half a cell along the E
+            // component's own axis, their Yee position (the reference: the
+            // cell corner).  Independent per record: computed on host threads.
+            struct Inc { double dE, pE, dH, pH; };
+            std::vector<Inc> inc(2 * recs.size());
+            host_parallel([&](int t, int nth) {
+                const size_t r0 = recs.size() * t / nth, r1 = recs.size() * (t + 1) / nth;
+                for (size_t q = r0; q < r1; q++) {
+                    const Rec &r = recs[q];
+                    for (int c = 0; c < 2; c++) {
+                        if (!(r.part & (1 << c))) continue;
+                        double o[3] = {0, 0, 0};
+                        if (m.flags & CFD_FIX_TFSF_YEE) o[compE[a][c]] = 0.5;
+                        const double x = m.dx * (r.i - i0 + o[0]), y = m.dy * (r.j - j0 + o[1]), z = m.dz * (r.k - k0 + o[2]);
+                        double xe = x, ye = y, ze = z;
+                        if (a == 0) xe = m.dx * (r.i - i0 + sh);
+                        if (a == 1) ye = m.dy * (r.j - j0 + sh);
+                        if (a == 2) ze = m.dz * (r.k - k0 + sh);
+                        Inc &v = inc[2 * q + c];
+                        at(xe, ye, ze, v.dE, v.pE);
+                        at(x, y, z, v.dH, v.pH);
+                    }
+                }
+            });
+            for (size_t q = 0; q < recs.size(); q++) {
+                const Rec &r = recs[q];
                 int hi = r.i, hj = r.j, hk = r.k;
                 const int off = e ? far : -1;
                 if (a == 0) hi += off; else if (a == 1) hj += off; else hk += off;
                 for (int c = 0; c < 2; c++) {
                     if (!(r.part & (1 << c))) continue;
-                    // incident values of this component pair: the H one (for the E
-                    // correction) half a cell across the face, the E one on it; with
-                    // CFD_FIX_TFSF_YEE both also half a cell along the E component's
-                    // own axis, their Yee position (the reference: the cell corner)
-                    double o[3] = {0, 0, 0};
-                    if (m.flags & CFD_FIX_TFSF_YEE) o[compE[a][c]] = 0.5;
-                    const double x = m.dx * (r.i - i0 + o[0]), y = m.dy * (r.j - j0 + o[1]), z = m.dz * (r.k - k0 + o[2]);
-                    double xe = x, ye = y, ze = z;
-                    if (a == 0) xe = m.dx * (r.i - i0 + sh);
-                    if (a == 1) ye = m.dy * (r.j - j0 + sh);
-                    if (a == 2) ze = m.dz * (r.k - k0 + sh);
-                    double dE, pE, dH, pH;
-                    at(xe, ye, ze, dE, pE);
-                    at(x, y, z, dH, pH);
+                    const double dE = inc[2 * q + c].dE, pE = inc[2 * q + c].pE;
+                    const double dH = inc[2 * q + c].dH, pH = inc[2 * q + c].pH;
                     const double s = sgn[a][c] * flip;
                     const double cE = s < 0 ? -ampE[a][c] : ampE[a][c], cH = s < 0 ? -ampH[a][c] : ampH[a][c];
                     add(false, compE[a][c], r.i, r.j, r.k, cE, dE, pE);
@@ -678,15 +720,28 @@ Port load_port(const Model &m, const char *path, int kind) {
     return P;
 }
 
+// dispersive and gain cells in (i, j, k) order: ranges of i scanned on host
+// threads, their lists joined in i order
 void cell_lists(Model &m) {
+    if (!m.has_dl && !m.has_gain) return;
     const int ie0 = m.pbx ? 0 : 1, je0 = m.pby ? 0 : 1;
-    for (int i = ie0; i < m.Nx; i++)
-        for (int j = je0; j < m.Ny; j++)
-            for (int k = 1; k < m.Nz; k++) {
-                const size_t p = m.flat(i, j, k);
-                if (m.has_dl && m.id[p] == 2) m.dl_cells.push_back((long long)p);
-                if (m.has_gain && m.id[p] == 5) m.gain_cells.push_back((long long)p);
+    std::vector<std::vector<long long>> dl(8), gain(8);
+    host_parallel([&](int t, int nth) {
+        const int a = ie0 + (int)((long long)(m.Nx - ie0) * t / nth), b = ie0 + (int)((long long)(m.Nx - ie0) * (t + 1) / nth);
+        for (int i = a; i < b; i++)
+            for (int j = je0; j < m.Ny; j++) {
+                const size_t p0 = m.flat(i, j, 0);
+                for (int k = 1; k < m.Nz; k++) {
+                    const int id = m.id[p0 + k];
+                    if (m.has_dl && id == 2) dl[t].push_back((long long)(p0 + k));
+                    if (m.has_gain && id == 5) gain[t].push_back((long long)(p0 + k));
+                }
             }
+    });
+    for (int t = 0; t < 8; t++) {
+        m.dl_cells.insert(m.dl_cells.end(), dl[t].begin(), dl[t].end());
+        m.gain_cells.insert(m.gain_cells.end(), gain[t].begin(), gain[t].end());
+    }
 }
 
 }  // namespace
@@ -735,14 +790,16 @@ Model build_model(const cfd_config &cfg) {
         const int nf = cfg.n_freq;
         m.twE.resize((size_t)nf * m.nt * 2);
         m.twH.resize((size_t)nf * m.nt * 2);
-        for (int f = 0; f < nf; f++)
-            for (int n = 0; n < m.nt; n++) {
-                const double tE = ((double)n + 1.0) * m.dt, tH = ((double)n + 0.5) * m.dt;
-                const double aE = 2.0 * kPi * m.freqs[f] * tE, aH = 2.0 * kPi * m.freqs[f] * tH;
-                const size_t o = ((size_t)f * m.nt + n) * 2;
-                m.twE[o] = std::cos(aE); m.twE[o + 1] = std::sin(aE);
-                m.twH[o] = std::cos(aH); m.twH[o + 1] = std::sin(aH);
-            }
+        host_parallel([&](int t, int nth) {  // frequencies shared among host threads
+            for (int f = t; f < nf; f += nth)
+                for (int n = 0; n < m.nt; n++) {
+                    const double tE = ((double)n + 1.0) * m.dt, tH = ((double)n + 0.5) * m.dt;
+                    const double aE = 2.0 * kPi * m.freqs[f] * tE, aH = 2.0 * kPi * m.freqs[f] * tH;
+                    const size_t o = ((size_t)f * m.nt + n) * 2;
+                    m.twE[o] = std::cos(aE); m.twE[o + 1] = std::sin(aE);
+                    m.twH[o] = std::cos(aH); m.twH[o + 1] = std::sin(aH);
+                }
+        });
     }
     for (int q = 0; q < cfg.n_ports; q++) m.ports.push_back(load_port(m, cfg.port_files[q], cfg.port_kinds ? cfg.port_kinds[q] : 0));
     return m;

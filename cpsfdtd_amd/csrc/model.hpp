@@ -6,9 +6,11 @@
 // bit-identical to the reference's) and then uploaded by engine.hip.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cpsfdtd.h"
@@ -18,6 +20,18 @@ namespace cfd {
 struct Error : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+
+// f(t, nth) on nth host threads (at most 8; t = 0 runs on the caller's), for
+// set-up loops whose parts are independent; f splits its own work by t.  f
+// must not throw.
+template <class F>
+void host_parallel(F &&f) {
+    const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back([&f, t, nth] { f(t, nth); });
+    f(0, nth);
+    for (std::thread &th : pool) th.join();
+}
 
 // Constants.h:19-26
 constexpr double kPi = 3.1415926535897932384626433832795;
