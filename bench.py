@@ -333,7 +333,7 @@ def batched_c4(steps=200, warmup=10, n=512):
             "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()}}
 
 
-def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack_cells=0):
+def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack_cells=0, step_cells=0):
     """The first ``nreq`` requests of the serving workload's stream (Poisson
     20/s, log-uniform 32-320 edges, 500-8000 steps, sphere / Drude-Lorentz /
     gain mix), submitted at their arrival times to the server: latency from
@@ -349,7 +349,7 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack
         os.makedirs(d)
         cases.append((t, configs.c5(d, p)))
     work = sum(p["N"][0] * p["N"][1] * p["N"][2] * p["steps"] for _, p in rq)
-    srv = eng.Server(quantum=quantum, policy=policy, max_active=max_active, pack_cells=pack_cells)
+    srv = eng.Server(quantum=quantum, policy=policy, max_active=max_active, pack_cells=pack_cells, step_cells=step_cells)
     try:
         t0 = time.perf_counter()
         ids = []
@@ -444,7 +444,7 @@ def serving_c5_kernels(nreq=40, precision=32):
                         for k, v in sorted(tot.items(), key=lambda kv: -kv[1][0])}}
 
 
-def serving_c2(nreq=4, precision=32, steps=4000):
+def serving_c2(nreq=4, precision=32, steps=4000, step_cells=0):
     """``nreq`` copies of the 256^3 workload (the whole 4000-step run each)
     submitted at once to the server with its defaults: served rate over the
     makespan and latency from submission to port results on the host, beside
@@ -453,7 +453,7 @@ def serving_c2(nreq=4, precision=32, steps=4000):
     from cpsfdtd_amd import configs
     cases = [configs.c2(tempfile.mkdtemp(prefix="bench_c2s_"), nt=steps) for _ in range(nreq)]
     cells = 256 ** 3
-    srv = eng.Server()
+    srv = eng.Server(step_cells=step_cells)
     try:
         t0 = time.perf_counter()
         ids = [srv.submit(c, precision) for c in cases]

@@ -583,7 +583,7 @@ def dft_flux(dft, axis, scale):
 
 class _ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("quantum", C.c_int), ("policy", C.c_int),
-                ("pack_cells", C.c_double)]
+                ("pack_cells", C.c_double), ("step_cells", C.c_double)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -594,8 +594,8 @@ class Server:
     """Serving loop (cfd_server_*): submit cases, then poll or wait; results
     are the port |E|^2 series and DFTs, on the host once a request is done."""
 
-    def __init__(self, hbm_budget=0, max_active=0, quantum=0, policy=0, pack_cells=0):
-        o = _ServerOpts(hbm_budget, max_active, quantum, policy, float(pack_cells))
+    def __init__(self, hbm_budget=0, max_active=0, quantum=0, policy=0, pack_cells=0, step_cells=0):
+        o = _ServerOpts(hbm_budget, max_active, quantum, policy, float(pack_cells), float(step_cells))
         h = C.c_void_p()
         if library().cfd_server_create(C.byref(o), C.byref(h)) < 0:
             raise EngineError(library().cfd_server_last_error().decode())

@@ -229,6 +229,9 @@ typedef struct {
     int quantum;         /* steps per scheduling slice (0: 256) */
     int policy;          /* CFD_POLICY_* (0: FIFO) */
     double pack_cells;   /* requests with fewer cells are packed (0: none) */
+    double step_cells;   /* stream requests stepping at once hold at most this many
+                            cells together, the earliest admitted first; one always
+                            steps (0: 2e7; < 0: no limit) */
 } cfd_server_opts;
 typedef struct cfd_server cfd_server;
 int cfd_server_create(const cfd_server_opts *opts, cfd_server **out);

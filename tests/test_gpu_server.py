@@ -32,9 +32,9 @@ def make(root, k, params):
     return configs.c5(d, params)
 
 
-@pytest.mark.parametrize("pack_cells", [1e9, 0], ids=["packed", "streams"])
-def test_server_results_equal_solo_runs(tmp_path, pack_cells):
-    srv = eng.Server(max_active=3, quantum=37, pack_cells=pack_cells)
+@pytest.mark.parametrize("pack_cells,step_cells", [(1e9, 0), (0, 0), (0, 1)], ids=["packed", "streams", "one-stepping"])
+def test_server_results_equal_solo_runs(tmp_path, pack_cells, step_cells):
+    srv = eng.Server(max_active=3, quantum=37, pack_cells=pack_cells, step_cells=step_cells)
     cases, ids = [], []
     for k, (p, prec) in enumerate(SMALL):
         case = make(tmp_path, k, p)
@@ -45,6 +45,7 @@ def test_server_results_equal_solo_runs(tmp_path, pack_cells):
         assert srv.wait(rid, 120.0) == eng.REQ_DONE, srv.error(rid)
     st = srv.stats()
     assert st["peak_active"] <= 3 and st["steps_launched"] == sum(p["steps"] for p, _ in SMALL)
+    done = []
     for rid, (case, prec) in zip(ids, cases):
         e = eng.Engine(case, prec)
         e.step(case.nt)
@@ -53,8 +54,11 @@ def test_server_results_equal_solo_runs(tmp_path, pack_cells):
         e.close()
         _, (t_sub, t_start, t_done) = srv.poll(rid)
         assert 0 <= t_sub <= t_start <= t_done
+        done.append(t_done)
         srv.release(rid)
     srv.close()
+    if step_cells:  # one request steps at a time, the earliest admitted first
+        assert done == sorted(done), done
 
 
 def test_server_isolates_a_failing_request(tmp_path):
