@@ -61,17 +61,17 @@ __device__ __forceinline__ bool special_id(const Dev<R> &d, int id)
     return ((id == 2) & (d.ndl != 0)) | ((id == 5) & (d.ngain != 0));
 }
 
-// Coefficient of medium id.  FP32 with nsel: a three-level select on the ID's
-// bits over the values held in the kernel argument, so the update issues no
-// load that depends on the ID byte (a table load is a second memory round
-// trip after the field loads).  Otherwise the table: in FP64 the selects on
-// register pairs cost the shell and pack kernels a wave per SIMD.
+// Coefficient of medium id.  With nsel (at most kSelMat media): a three-level
+// select on the ID's bits over the values held in the kernel argument, so the
+// update issues no load that depends on the ID byte (a table load is a second
+// memory round trip after the field loads; 256^3 FP64 merged update kernels
+// 273 -> 261 us with the selects).  Otherwise the table.
 // SEL = false: always the table (kernels whose simulation view is read from
 // device memory, where the selected values would be 32 more scalar loads).
 template <bool SEL, class R>
 __device__ __forceinline__ R mat_coef(const R (&v)[kSelMat], const R *tab, int nsel, int id)
 {
-    if (SEL && sizeof(R) == 4 && nsel) {
+    if (SEL && nsel) {
         const bool b0 = id & 1, b1 = id & 2, b2 = id & 4;
         const R v01 = b0 ? v[1] : v[0], v23 = b0 ? v[3] : v[2], v45 = b0 ? v[5] : v[4], v67 = b0 ? v[7] : v[6];
         const R v03 = b1 ? v23 : v01, v47 = b1 ? v67 : v45;
