@@ -516,14 +516,24 @@ public:
             // one thread per corrected value; targets sorted by (component,
             // device address) so a wavefront's read-modify-writes are contiguous
             // (each target keeps its terms in the reference's face order)
+            // (the E and H lists on two host threads, each sorted by a 64-bit
+            // key: component above the device index, which is below 2^40)
             std::vector<TfTarget> te, th;
-            for (const TfsfTarget &t : m.tfE) te.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
-            for (const TfsfTarget &t : m.tfH) th.push_back({dev_index(t.i, t.j, t.k), t.comp, t.first, t.count, 0});
-            auto by_addr = [](const TfTarget &a, const TfTarget &b) {
-                return a.comp != b.comp ? a.comp < b.comp : a.cell < b.cell;
-            };
-            std::sort(te.begin(), te.end(), by_addr);
-            std::sort(th.begin(), th.end(), by_addr);
+            host_parallel([&](int t, int) {
+                if (t > 1) return;
+                const std::vector<TfsfTarget> &src = t ? m.tfH : m.tfE;
+                std::vector<std::pair<unsigned long long, unsigned>> key(src.size());
+                for (size_t q = 0; q < src.size(); q++)
+                    key[q] = {(unsigned long long)src[q].comp << 60 | (unsigned long long)dev_index(src[q].i, src[q].j, src[q].k),
+                              (unsigned)q};
+                std::sort(key.begin(), key.end());
+                std::vector<TfTarget> &dst = t ? th : te;
+                dst.reserve(src.size());
+                for (const auto &kq : key) {
+                    const TfsfTarget &g = src[kq.second];
+                    dst.push_back({dev_index(g.i, g.j, g.k), g.comp, g.first, g.count, 0});
+                }
+            });
             // terms laid out in sorted-target order too
             std::vector<double> c, dl, pr;
             for (std::vector<TfTarget> *tv : {&te, &th})
@@ -1243,7 +1253,15 @@ public:
                 const long long t0 = c * chunk;
                 const int n = (int)std::min(chunk, np_ - t0);
                 HIP_OK(hipEventSynchronize(ev[c & 1]));
-                std::memcpy(host + t0 * rows, pin[c & 1], (size_t)n * rows * sizeof(double));
+                // split over host threads: the first touch of fresh result
+                // pages (page faults) dominates a single-threaded copy
+                const size_t bytes = (size_t)n * rows * sizeof(double);
+                char *dst = reinterpret_cast<char *>(host + t0 * rows);
+                const char *src = reinterpret_cast<const char *>(pin[c & 1]);
+                host_parallel([&](int t, int nth) {
+                    const size_t a = bytes * t / nth / 64 * 64, b = t + 1 == nth ? bytes : bytes * (t + 1) / nth / 64 * 64;
+                    std::memcpy(dst + a, src + a, b - a);
+                });
             };
             for (long long c = 0; c < nch; c++) {
                 const long long t0 = c * chunk;
