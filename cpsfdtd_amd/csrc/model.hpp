@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <exception>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -22,15 +23,31 @@ struct Error : std::runtime_error {
 };
 
 // f(t, nth) on nth host threads (at most 8; t = 0 runs on the caller's), for
-// set-up loops whose parts are independent; f splits its own work by t.  f
-// must not throw.
+// set-up loops whose parts are independent; f splits its own work by t.  An
+// exception thrown by any part is rethrown here once every part has ended.
 template <class F>
 void host_parallel(F &&f) {
     const int nth = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::exception_ptr> err(nth);
+    auto part = [&](int t) {
+        try {
+            f(t, nth);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
     std::vector<std::thread> pool;
-    for (int t = 1; t < nth; t++) pool.emplace_back([&f, t, nth] { f(t, nth); });
-    f(0, nth);
+    for (int t = 1; t < nth; t++) {
+        try {
+            pool.emplace_back(part, t);
+        } catch (...) {  // no thread to be had: this part on the caller's
+            part(t);
+        }
+    }
+    part(0);
     for (std::thread &th : pool) th.join();
+    for (const std::exception_ptr &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // Constants.h:19-26
