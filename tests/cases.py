@@ -16,15 +16,19 @@ REF_RESOURCES = "/root/reference/resourses"
 
 def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZEROHEAP | SRC_REF_PULSE,
           zs=0.0, z0=0.0, theta=30.0, phi=45.0, psi=90.0, src=(12, 14, 16), dl=False, gain=False, dl_medium=None,
-          record=True, freqs=(), stride=1, periodic=0, cw=0, magnetic=False):
+          record=True, freqs=(), stride=1, periodic=0, cw=0, magnetic=False, extra_media=0):
     """Every shape painter, CPML on all faces, an oblique Gaussian-beam TF/SF
     (cw=1: the continuous-wave form), a soft Ez source, a plane and a sphere
     port.  magnetic: media 1 and 3 get mu_r 2 and 1.5, medium 3 a magnetic
-    conductivity of 1e9 ohm/m (sigma_M dt / 2 mu0 mu_r about 0.5 %)."""
+    conductivity of 1e9 ohm/m (sigma_M dt / 2 mu0 mu_r about 0.5 %).
+    extra_media: that many more dielectrics after the six, the last painted
+    as a small block (more than eight media: the engine's coefficient tables
+    instead of its selects)."""
     d = (10e-9, 10e-9, 10e-9)
     w = CaseWriter(directory)
     w.materials([(1, 1, 0, 0), (2.25, 2.0 if magnetic else 1, 0, 0), (1, 1, 0, 0),
-                 (4.0, 1.5 if magnetic else 1, 0.01, 1e9 if magnetic else 0), (1, 1, 0, 0), (1.5, 1, 0, 0)])
+                 (4.0, 1.5 if magnetic else 1, 0.01, 1e9 if magnetic else 0), (1, 1, 0, 0), (1.5, 1, 0, 0)] +
+                [(1.2 + 0.3 * q, 1, 0.002 * q, 0) for q in range(extra_media)])
     sx, sy, sz = n[0] / 40.0, n[1] / 36.0, n[2] / 32.0
     w.grid(n, d, [
         Shape(1, "ellipsoid", x0=200e-9 * sx, y0=180e-9 * sy, z0=160e-9 * sz, a=80e-9, b=70e-9, c=60e-9),
@@ -39,7 +43,9 @@ def mixed(directory, n=(40, 36, 32), npml=(6, 5, 6), box=9, nt=60, flags=REF_ZER
     ] + ([  # dispersive / gain blocks reaching into CPML corners
         Shape(2, "cuboid", x0=40e-9, y0=180e-9 * sy, z0=40e-9, a=60e-9, b=50e-9, c=60e-9)] if dl else []) + ([
         Shape(5, "cuboid", x0=(n[0] - 4) * 10e-9, y0=40e-9, z0=(n[2] - 4) * 10e-9, a=60e-9, b=60e-9, c=60e-9)]
-        if gain else []))
+        if gain else []) + ([
+        Shape(5 + extra_media, "cuboid", x0=80e-9 * sx, y0=200e-9 * sy, z0=60e-9 * sz, a=70e-9, b=60e-9, c=80e-9)]
+        if extra_media else []))
     if npml:
         w.cpml(((npml[0], npml[0]), (npml[1], npml[1]), (npml[2], npml[2])), m=3.0, malp=1.0, kapmax=5.0,
                sig_ratio=1.0, alpmax=0.05)

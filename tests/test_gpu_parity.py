@@ -85,6 +85,17 @@ def test_update_forms_bit_exact(tmp_path, monkeypatch, form):
     assert np.array_equal(e.gain_pop(), o.gain_pop())
 
 
+@pytest.mark.parametrize("form", ["merged", "split"])
+def test_coefficient_tables_beyond_eight_media_f64_bit_exact(tmp_path, monkeypatch, form):
+    """Ten media: the update kernels read each medium's coefficients from
+    their tables (up to eight they select them from the kernel arguments)."""
+    monkeypatch.setenv("CFD_UPDATE_KERNEL", form)
+    case = cases.mixed(str(tmp_path), dl=True, flags=FIXED, extra_media=4)
+    e, o = run_both(case, 40)
+    assert_fields_equal(e, o, form)
+    assert np.array_equal(e.port_samples(0), o.port_samples(0))
+
+
 def test_dl_gain_f64_bit_exact(tmp_path):
     case = cases.mixed(str(tmp_path), dl=True, gain=True, flags=FIXED)
     e, o = run_both(case, 50)
