@@ -884,12 +884,17 @@ __device__ __forceinline__ void h_tile(const Dev<R> &d, const MTile &t, const Bo
     st_once<NT>(&d.F[5][p], hz);
 }
 
+// The H pass runs the planes (and a batch's instances) in descending order,
+// the E pass in ascending order: each pass starts where the one before it
+// ended, on the planes whose lines the Infinity Cache still holds (256^3:
+// FP32 step 0.2806 -> 0.2782 ms, FP64 0.583 -> 0.570 ms).  Cells of one
+// pass are independent, so the order changes no value.
 template <class R, bool CS, bool NT, bool B = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFullWaves<R, B>))) void k_full_h(Dev<R> d0, const Dev<R> *devs, MGrid g, Box L, LeanStretch<R> c)
 {
-    const Dev<R> &d = instance_z<R, B>(d0, devs);
+    const Dev<R> &d = B ? devs[gridDim.z - 1 - blockIdx.z] : d0;
     MTile t;
-    mtile_at(g, blockIdx.x, g.k_lo + (int)blockIdx.y, t);
+    mtile_at(g, blockIdx.x, g.k_lo + (g.planes - 1 - (int)blockIdx.y), t);
     h_tile<R, CS, NT, !B>(d, t, L, c);
 }
 
