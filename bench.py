@@ -383,8 +383,10 @@ def serving_c5(nreq=120, precision=32, quantum=256, policy=0, max_active=0, pack
             "demand_mcells": round(work / rq[-1][0] / 1e6, 1), "peak_active": stats["peak_active"],
             "peak_device_gb": round(stats["peak_bytes"] / 1e9, 2), "quantum": quantum,
             "policy": "shortest work first" if policy == eng.POLICY_SHORTEST else "first come, first served",
-            "stepping": "requests under %g cells packed (one launch per pass over all of them), larger ones on their "
-            "own streams" % pack_cells if pack_cells else "one stream per request"}
+            "stepping": ("requests under %g cells packed (one launch per pass over all of them), larger ones on their "
+                         "own streams" % pack_cells if pack_cells else "one stream per request")
+                        + ("; at most %g cells stepping at once" % (step_cells if step_cells > 0 else 2e7)
+                           if step_cells >= 0 else "")}
 
 
 def serving_c5_sequential(nreq=40, precision=32):
@@ -480,6 +482,8 @@ def serving_c2(nreq=4, precision=32, steps=4000, step_cells=0):
             "makespan_s": round(span, 2), "p50_s": round(float(np.percentile(lat, 50)), 3),
             "max_s": round(max(lat), 3), "peak_active": stats["peak_active"],
             "solo_sequential": round(nreq * cells * steps / solo / 1e6, 2),
+            "stepping": "one stream per request; at most %g cells stepping at once" % (step_cells if step_cells > 0 else 2e7)
+            if step_cells >= 0 else "one stream per request",
             "basis": "server: setup, stepping, results to host and release inside the makespan; solo: the same "
                      "per request (Engine set-up, steps, DFT read back, close)"}
 
