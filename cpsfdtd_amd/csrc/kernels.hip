@@ -1254,12 +1254,15 @@ __global__ __launch_bounds__(256) void k_post_e(Dev<R> d, PostE<R> P, const int 
     if (nbase) n += *nbase;  // replayed step blocks: step = block start + offset
     const int b = blockIdx.x;
     if (b < P.b1) {
+        // the special cells and the TF/SF-E targets from the top planes
+        // down: the E pass before this launch ended there (256^3 FP32:
+        // 21.7 -> 21.1 us)
         if constexpr (ND >= -1) {
-            const int t = b * 256 + threadIdx.x;
+            const int t = (P.b1 - 1 - b) * 256 + threadIdx.x;
             if (t < P.nspecial) special_cell<R, ND, NL>(d, P.special[t]);
         }
     } else if (b < P.b2) {
-        const int t = (b - P.b1) * 256 + threadIdx.x;
+        const int t = (P.b2 - 1 - b) * 256 + threadIdx.x;
         if (t >= P.ntg) return;
         if constexpr (std::is_same<R, float>::value) {
             if (P.tg32) {
